@@ -1,0 +1,163 @@
+/*
+ * spike_pstrsv for AMD Instinct MI355X (gfx950) -- public C ABI.
+ *
+ * Drop-in replacement for the reference library's public interface
+ * (src/spike_pstrsv.h in the reference repository).  The five legacy entry
+ * points keep their exact C signatures; MKL_INT is a 32-bit integer, as in the
+ * reference's LP64 MKL link (Makefile.sample:15).
+ *
+ * Matrices are zero-based 3-array CSR (ia[0] == 0, columns ascending, no empty
+ * rows), with the diagonal stored as the first entry of each row for
+ * uplo 'U'/'u' and as the last entry for 'L'/'l' (reference
+ * src/spike_pstrsv_ops.c:1186 and :1277).
+ *
+ * Differences in behaviour from the reference, all compatible with its callers:
+ *   - 'L' is solved (the reference leaves x untouched for 'L');
+ *   - b is never modified (the reference uses it as scratch);
+ *   - the number of Spike partitions is independent of nthreads; by default
+ *     it equals nthreads, and SPIKE_PSTRSV_PARTITIONS or
+ *     spike_pstrsv_set_partitions() override it;
+ *   - errors never exit(): the legacy void functions print one line to stderr
+ *     and leave x untouched; the extension API returns status codes;
+ *   - host and device pointers are both accepted for b and x.
+ */
+#ifndef SPIKE_PSTRSV_H
+#define SPIKE_PSTRSV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef MKL_INT
+#define MKL_INT int
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Legacy entry points (one implicit, library-owned current factor)    */
+/* ------------------------------------------------------------------ */
+
+/* Replaces spike_pstrsv_preproc, reference src/spike_pstrsv.h:7
+ * (implementation src/spike_pstrsv.c:40-81).  Note: uplo is the LAST argument. */
+void spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo);
+
+/* Replaces _spike_pstrsv_preproc, reference src/spike_pstrsv.h:14
+ * (src/spike_pstrsv.c:83-140): same as spike_pstrsv_preproc, and records
+ * per-phase wall times (see spike_pstrsv_get_timers). */
+void _spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo);
+
+/* Replaces spike_pstrsv, reference src/spike_pstrsv.h:21
+ * (src/spike_pstrsv.c:142-147): preprocess, then solve. */
+void spike_pstrsv(const char uplo, int m, double *a, MKL_INT *ia, MKL_INT *ja, double *b, double *x, int nthreads);
+
+/* Replaces spike_pstrsv_solve, reference src/spike_pstrsv.h:31
+ * (src/spike_pstrsv.c:149-239): solves with the current factor. */
+void spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthreads);
+
+/* Replaces _spike_pstrsv_solve, reference src/spike_pstrsv.h:38
+ * (src/spike_pstrsv.c:241-355): same as spike_pstrsv_solve, and records
+ * per-phase device times (see spike_pstrsv_get_timers). */
+void _spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthreads);
+
+/* ------------------------------------------------------------------ */
+/* Extension API (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------ */
+
+/* Status codes returned by the extension API. */
+enum {
+    SPIKE_OK = 0,
+    SPIKE_ERR_ARG = 1,        /* invalid argument (uplo, n, nrhs, ld, pointers) */
+    SPIKE_ERR_MATRIX = 2,     /* CSR violates the conventions above            */
+    SPIKE_ERR_DEVICE = 3,     /* HIP runtime error / no usable gfx950 device   */
+    SPIKE_ERR_NOMEM = 4,      /* device or host allocation failed              */
+    SPIKE_ERR_STATE = 5,      /* e.g. solve before preprocess, uplo mismatch   */
+    SPIKE_ERR_UNSUPPORTED = 6 /* configuration outside the implemented range   */
+};
+
+/* Precision modes of a factor. */
+enum {
+    SPIKE_PREC_FP64 = 0       /* FP64 values and arithmetic throughout         */
+};
+
+typedef struct spike_factor_s *spike_factor_t;
+
+typedef struct {
+    int partitions;           /* Spike partition count P; 0 = automatic        */
+    double drop_tol;          /* reduced-system drop tolerance on the spike    */
+                              /* row sum (sum of dropped |w| <= drop_tol per   */
+                              /* row); < 0 = default (1e-15), 0 = keep all     */
+                              /* nonzeros (as the reference)                   */
+    int precision;            /* SPIKE_PREC_*                                  */
+    int lanes_per_row;        /* block-solve lanes per row (1..64, power of 2);*/
+                              /* 0 = automatic                                 */
+    int verbose;              /* print the partition log after preprocessing   */
+} spike_options_t;
+
+/* Fills opts with the defaults. */
+void spike_options_default(spike_options_t *opts);
+
+/* Analyses and factors the CSR matrix (host pointers).  The matrix is copied;
+ * the caller keeps ownership of a/ia/ja. */
+int spike_factor_create(spike_factor_t *f, char uplo, int n, const double *a,
+                        const int *ia, const int *ja, const spike_options_t *opts);
+
+/* Solves A X = B for nrhs right-hand sides (column-major, leading dimensions
+ * ldb/ldx >= n).  b and x may be host or device pointers.  stream is a
+ * hipStream_t (NULL = default stream); for device pointers the call is
+ * asynchronous with respect to the host, for host pointers it synchronises. */
+int spike_factor_solve(spike_factor_t f, int nrhs, const double *b, int ldb,
+                       double *x, int ldx, void *stream);
+
+int spike_factor_destroy(spike_factor_t f);
+
+typedef struct {
+    int n;
+    int partitions;           /* effective P                                   */
+    int64_t nnz;
+    int n_reduced;            /* dependents = reduced-system size              */
+    int64_t nnz_reduced;      /* kept off-diagonals + n_reduced                */
+    int reduced_levels;       /* dependency levels of the reduced system       */
+    int64_t spike_area;       /* sum of spike block sizes (rows x cols)        */
+    double dropped_rowsum_max;/* largest sum of dropped |w| over reduced rows  */
+    int steps_pass1;          /* longest block-solve step chain, pass 1        */
+    int steps_pass2;          /* longest block-solve step chain, pass 2        */
+    int lanes_per_row;        /* block-solve lanes per row in use              */
+    int64_t device_bytes;     /* device memory held by the factor              */
+    double preproc_seconds;   /* wall time of spike_factor_create              */
+} spike_stats_t;
+
+int spike_factor_stats(spike_factor_t f, spike_stats_t *st);
+
+/* Partition log in the reference's logPSTRSV3 field order
+ * (src/spike_pstrsv_ops.c:1311-1333), written to stdout. */
+int spike_factor_log(spike_factor_t f);
+
+/* Host-only symbolic analysis (no device needed): fills st with the sizes the
+ * factorisation would have, except fields that need the spikes (nnz_reduced,
+ * reduced_levels, dropped_rowsum_max, device_bytes are 0). */
+int spike_analyze(char uplo, int n, const double *a, const int *ia, const int *ja,
+                  const spike_options_t *opts, spike_stats_t *st);
+
+/* Partition count used by the legacy API when > 0 (overrides nthreads and the
+ * SPIKE_PSTRSV_PARTITIONS environment variable); 0 restores the default. */
+void spike_pstrsv_set_partitions(int P);
+
+/* Factor behind the legacy API (NULL before the first preprocess). */
+spike_factor_t spike_pstrsv_current_factor(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *spike_last_error(void);
+
+/* Per-phase timers of the profiling twins, in seconds:
+ * t[0] preprocess total, t[1] symbolic analysis, t[2] device spike build,
+ * t[3] reduced-system assembly, t[4] solve total (device),
+ * t[5] pass 1 block solve, t[6] reduced solve, t[7] coupling update,
+ * t[8] pass 2 block solve.  Returns the number of timers written (<= cap). */
+int spike_pstrsv_get_timers(double *t, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPIKE_PSTRSV_H */

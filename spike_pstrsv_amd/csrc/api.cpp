@@ -1,0 +1,640 @@
+// C ABI of spike_pstrsv for MI355X: the reference's five legacy entry points
+// (reference src/spike_pstrsv.h:7-38) over a device-resident factor object,
+// plus the extension API declared in include/spike_pstrsv.h.
+#include "spike_pstrsv.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using spike::DevPart;
+using spike::Plan;
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string &msg)
+{
+    t_err = msg;
+    return code;
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? SPIKE_ERR_NOMEM : SPIKE_ERR_DEVICE,        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+// Global timers of the profiling twins (seconds, accumulated like the
+// reference's timers in src/spike_pstrsv.c:18-37).
+double g_timers[9];
+
+}  // namespace
+
+struct spike_factor_s {
+    Plan plan;
+    spike_options_t opts{};
+    spike_stats_t stats{};
+    int device = 0;
+    std::vector<void *> allocs;
+    int64_t device_bytes = 0;
+
+    DevPart *d_parts = nullptr;
+    int *d_dep_rows = nullptr;
+    int *d_cpl_rows = nullptr, *d_cpl_ptr = nullptr, *d_cpl_col = nullptr;
+    double *d_cpl_val = nullptr;
+    spike::DevSteps s1{}, s2{};
+    int *d_rrp = nullptr, *d_rci = nullptr, *d_rlist = nullptr, *d_levptr = nullptr;
+    double *d_rv = nullptr;
+    int nlev = 0;
+    bool red_in_lds = false;
+    double *d_xred = nullptr, *d_bc = nullptr;
+    // staging for host right-hand sides / solutions
+    double *d_stage_b = nullptr, *d_stage_x = nullptr;
+    size_t stage_cap = 0;
+
+    template <class T>
+    int dalloc(T **p, size_t count)
+    {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SPIKE_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        allocs.push_back(*p);
+        device_bytes += (int64_t)(count * sizeof(T));
+        return SPIKE_OK;
+    }
+    template <class T>
+    int upload(T **p, const std::vector<T> &v)
+    {
+        int rc = dalloc(p, v.size());
+        if (rc) return rc;
+        if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return SPIKE_OK;
+    }
+    void release()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+        allocs.clear();
+        device_bytes = 0;
+    }
+};
+
+namespace {
+
+int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps &ds)
+{
+    double *val, *mult;
+    uint16_t *col, *row;
+    int64_t *pb;
+    int rc;
+    if ((rc = f->upload(&val, ss.val)) || (rc = f->upload(&col, ss.col)) || (rc = f->upload(&row, ss.row)) ||
+        (rc = f->upload(&mult, ss.mult)) || (rc = f->upload(&pb, ss.part_begin)))
+        return rc;
+    ds.val = val; ds.col = col; ds.row = row; ds.mult = mult; ds.part_begin = pb;
+    ds.G = ss.G; ds.M = ss.M; ds.R = ss.R;
+    return SPIKE_OK;
+}
+
+// Frees a temporary device buffer that was registered with dalloc.
+void dfree(spike_factor_s *f, void *p, size_t bytes)
+{
+    auto it = std::find(f->allocs.begin(), f->allocs.end(), p);
+    if (it != f->allocs.end()) {
+        f->allocs.erase(it);
+        f->device_bytes -= (int64_t)bytes;
+    }
+    (void)hipFree(p);
+}
+
+int kmax_limit_for_device()
+{
+    size_t lds = spike::block_solve_lds_limit();
+    int lim = (int)(lds / sizeof(double)) - 1;
+    return std::min(lim, 16383);
+}
+
+int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int *ia, const int *ja)
+{
+    const double t0 = now_s();
+    int rc;
+    HIP_TRY(hipGetDevice(&f->device));
+    spike::PlanOptions po;
+    po.kmax_limit = kmax_limit_for_device();
+    int P = f->opts.partitions;
+    if (P <= 0) P = std::max(1, n / 1024);
+    P = std::min(P, n);
+    // raise P until the largest partition fits the LDS x segment
+    while (P < n && n - (n / P) * (P - 1) > po.kmax_limit) P++;
+    po.P = P;
+    po.lanes_per_row = f->opts.lanes_per_row;
+    std::string err = spike::build_plan(uplo, n, a, ia, ja, po, f->plan);
+    if (!err.empty()) return fail(SPIKE_ERR_MATRIX, err);
+    Plan &pl = f->plan;
+    const double t1 = now_s();
+    g_timers[1] += t1 - t0;
+
+    // ---- per-partition records, dependents, coupling rows
+    std::vector<DevPart> hp(P);
+    for (int t = 0; t < P; t++) {
+        const spike::Part &pt = pl.parts[t];
+        hp[t] = DevPart{pt.s, pt.k, pt.f, pt.hi1, pt.dbeg, pt.dcnt, pt.cbeg, pt.ccnt};
+    }
+    if ((rc = f->upload(&f->d_parts, hp)) || (rc = f->upload(&f->d_dep_rows, pl.dep_rows)) ||
+        (rc = f->upload(&f->d_cpl_rows, pl.cpl_rows)) || (rc = f->upload(&f->d_cpl_ptr, pl.cpl_ptr)) ||
+        (rc = f->upload(&f->d_cpl_col, pl.cpl_col)) || (rc = f->upload(&f->d_cpl_val, pl.cpl_val)))
+        return rc;
+    if ((rc = upload_steps(f, pl.pass1, f->s1)) || (rc = upload_steps(f, pl.pass2, f->s2))) return rc;
+
+    const int nred = (int)pl.dep_rows.size();
+    if ((rc = f->dalloc(&f->d_xred, nred)) || (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size()))) return rc;
+
+    // ---- spikes on the device
+    std::vector<spike::SpikePart> sp;
+    std::vector<int64_t> woff(P, -1);
+    int64_t area = 0;
+    for (int t = 0; t < P; t++) {
+        const spike::Part &pt = pl.parts[t];
+        if (pt.kcnt > 0 && pt.rows_S > 0) {
+            woff[t] = area;
+            sp.push_back(spike::SpikePart{pt.s, pt.f, pt.rows_S, pt.kbeg, pt.kcnt, area});
+            area += (int64_t)pt.rows_S * pt.kcnt;
+        }
+    }
+    std::vector<int> hrp(nred + 1, 0);
+    std::vector<int> hci;
+    double dropmax = 0.0;
+    int64_t rnnz = 0;
+    double t2 = now_s(), t3 = t2;
+    if (nred > 0) {
+        spike::SpikePart *d_sp;
+        int *d_kcols, *d_ia, *d_ja;
+        double *d_a, *d_W;
+        if ((rc = f->upload(&d_sp, sp)) || (rc = f->upload(&d_kcols, pl.kcols)) || (rc = f->upload(&d_ia, pl.ia)) ||
+            (rc = f->upload(&d_ja, pl.ja)) || (rc = f->upload(&d_a, pl.a)) || (rc = f->dalloc(&d_W, (size_t)area)))
+            return rc;
+        HIP_TRY(spike::launch_spike_build((int)sp.size(), d_sp, d_kcols, d_ia, d_ja, d_a, d_W, nullptr));
+        HIP_TRY(hipDeviceSynchronize());
+        t2 = now_s();
+        g_timers[2] += t2 - t1;
+
+        // ---- reduced-system assembly on the device
+        std::vector<int64_t> wrow(nred, -1);
+        std::vector<int> qk(nred, 0), qkb(nred, 0);
+        std::vector<double> tol(nred, -1.0);
+        std::vector<int> kridx(pl.kcols.size());
+        for (size_t j = 0; j < pl.kcols.size(); j++) kridx[j] = pl.ridx[pl.kcols[j]];
+        const double dt = f->opts.drop_tol < 0 ? 1e-15 : f->opts.drop_tol;
+        for (int t = 0; t < P; t++) {
+            const spike::Part &pt = pl.parts[t];
+            for (int q = pt.dbeg; q < pt.dbeg + pt.dcnt; q++) {
+                int r = pl.dep_rows[q] - pt.s;
+                if (woff[t] >= 0 && r >= pt.f && r <= pt.g) {
+                    wrow[q] = woff[t] + (int64_t)(r - pt.f) * pt.kcnt;
+                    qk[q] = pt.kcnt;
+                    qkb[q] = pt.kbeg;
+                    tol[q] = dt > 0 ? dt / pt.kcnt : -1.0;
+                }
+            }
+        }
+        int64_t *d_wrow;
+        int *d_qk, *d_qkb, *d_kridx, *d_cnt;
+        double *d_tol, *d_drop;
+        if ((rc = f->upload(&d_wrow, wrow)) || (rc = f->upload(&d_qk, qk)) || (rc = f->upload(&d_qkb, qkb)) ||
+            (rc = f->upload(&d_tol, tol)) || (rc = f->upload(&d_kridx, kridx)) || (rc = f->dalloc(&d_cnt, nred)) ||
+            (rc = f->dalloc(&d_drop, nred)))
+            return rc;
+        HIP_TRY(spike::launch_reduced_count(nred, d_wrow, d_qk, d_tol, d_W, d_cnt, d_drop, nullptr));
+        std::vector<int> cnt(nred);
+        std::vector<double> drop(nred);
+        HIP_TRY(hipMemcpy(cnt.data(), d_cnt, sizeof(int) * nred, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(drop.data(), d_drop, sizeof(double) * nred, hipMemcpyDeviceToHost));
+        for (int q = 0; q < nred; q++) {
+            rnnz += cnt[q];
+            if (rnnz > INT32_MAX) return fail(SPIKE_ERR_UNSUPPORTED, "reduced system exceeds 2^31-1 entries");
+            hrp[q + 1] = (int)rnnz;
+            dropmax = std::max(dropmax, drop[q]);
+        }
+        if ((rc = f->upload(&f->d_rrp, hrp)) || (rc = f->dalloc(&f->d_rci, (size_t)rnnz)) ||
+            (rc = f->dalloc(&f->d_rv, (size_t)rnnz)))
+            return rc;
+        HIP_TRY(spike::launch_reduced_fill(nred, d_wrow, d_qkb, d_qk, d_tol, d_W, d_kridx, f->d_rrp, f->d_rci,
+                                           f->d_rv, nullptr));
+        hci.resize(rnnz);
+        if (rnnz) HIP_TRY(hipMemcpy(hci.data(), f->d_rci, sizeof(int) * rnnz, hipMemcpyDeviceToHost));
+        HIP_TRY(hipDeviceSynchronize());
+        dfree(f, d_W, sizeof(double) * std::max<int64_t>(area, 1));
+        dfree(f, d_a, sizeof(double) * pl.a.size());
+        dfree(f, d_ja, sizeof(int) * pl.ja.size());
+        dfree(f, d_ia, sizeof(int) * pl.ia.size());
+        dfree(f, d_wrow, sizeof(int64_t) * nred);
+        dfree(f, d_cnt, sizeof(int) * nred);
+        dfree(f, d_drop, sizeof(double) * nred);
+        t3 = now_s();
+        g_timers[3] += t3 - t2;
+    }
+
+    // ---- level schedule of the reduced system (rows with off-diagonals)
+    std::vector<int> lev = spike::row_levels(nred, hrp.data(), hci.data());
+    int maxlev = 0;
+    for (int q = 0; q < nred; q++) maxlev = std::max(maxlev, lev[q]);
+    std::vector<int> levptr(maxlev + 1, 0), rlist;
+    for (int q = 0; q < nred; q++)
+        if (lev[q] > 0) levptr[lev[q]]++;
+    for (int L = 1; L <= maxlev; L++) levptr[L] += levptr[L - 1];
+    rlist.resize(levptr[maxlev]);
+    {
+        std::vector<int> fill(levptr.begin(), levptr.end() - 1);
+        for (int q = 0; q < nred; q++)
+            if (lev[q] > 0) rlist[fill[lev[q] - 1]++] = q;
+    }
+    f->nlev = maxlev;
+    if ((rc = f->upload(&f->d_levptr, levptr)) || (rc = f->upload(&f->d_rlist, rlist))) return rc;
+    f->red_in_lds = (size_t)nred * sizeof(double) <= spike::block_solve_lds_limit();
+
+    // host copies no longer needed
+    pl.a.clear(); pl.a.shrink_to_fit();
+    pl.ja.clear(); pl.ja.shrink_to_fit();
+    pl.pass1.val.clear(); pl.pass1.val.shrink_to_fit(); pl.pass1.col.clear(); pl.pass1.col.shrink_to_fit();
+    pl.pass2.val.clear(); pl.pass2.val.shrink_to_fit(); pl.pass2.col.clear(); pl.pass2.col.shrink_to_fit();
+    pl.cpl_val.clear(); pl.cpl_val.shrink_to_fit();
+
+    spike_stats_t &st = f->stats;
+    st.n = n;
+    st.partitions = P;
+    st.nnz = pl.nnz;
+    st.n_reduced = nred;
+    st.nnz_reduced = rnnz + nred;
+    st.reduced_levels = maxlev;
+    st.spike_area = pl.spike_area;
+    st.dropped_rowsum_max = dropmax;
+    st.steps_pass1 = pl.pass1.max_steps;
+    st.steps_pass2 = pl.pass2.max_steps;
+    st.lanes_per_row = pl.pass1.G;
+    st.device_bytes = f->device_bytes;
+    st.preproc_seconds = now_s() - t0;
+    g_timers[0] += st.preproc_seconds;
+    if (f->opts.verbose) spike_factor_log(f);
+    return SPIKE_OK;
+}
+
+bool is_device_ptr(const void *p)
+{
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.isManaged;
+}
+
+// One right-hand side, all pointers on the device.  ev (optional, 5 events)
+// brackets the phases for the profiling twin.
+int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipEvent_t *ev)
+{
+    const Plan &pl = f->plan;
+    spike::BlockArgs ba;
+    ba.parts = f->d_parts;
+    ba.b = b;
+    ba.x = x;
+    ba.xred = f->d_xred;
+    ba.bc = f->d_bc;
+    ba.dep_rows = f->d_dep_rows;
+    ba.cpl_rows = f->d_cpl_rows;
+    ba.n = pl.n;
+    ba.flip = pl.flip ? 1 : 0;
+    ba.kmax = pl.kmax;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+    ba.st = f->s1;
+    HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s));
+    const int nred = (int)pl.dep_rows.size();
+    if (nred > 0) {
+        HIP_TRY(spike::launch_reduced_solve(nred, f->nlev, f->d_levptr, f->d_rlist, f->d_rrp, f->d_rci, f->d_rv,
+                                            f->d_xred, f->red_in_lds, s));
+        if (ev) HIP_TRY(hipEventRecord(ev[2], s));
+        HIP_TRY(spike::launch_coupling((int)pl.cpl_rows.size(), f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
+                                       f->d_cpl_val, f->d_xred, b, f->d_bc, pl.n, ba.flip, s));
+        if (ev) HIP_TRY(hipEventRecord(ev[3], s));
+        ba.st = f->s2;
+        HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
+    } else if (ev) {
+        HIP_TRY(hipEventRecord(ev[2], s));
+        HIP_TRY(hipEventRecord(ev[3], s));
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[4], s));
+    return SPIKE_OK;
+}
+
+int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, int ldx, hipStream_t s,
+                 bool timed)
+{
+    const int n = f->plan.n;
+    if (nrhs < 0 || ldb < n || ldx < n) return fail(SPIKE_ERR_ARG, "nrhs must be >= 0 and ldb, ldx >= n");
+    if (nrhs == 0) return SPIKE_OK;
+    if (!b || !x) return fail(SPIKE_ERR_ARG, "b and x must be non-NULL");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != f->device) HIP_TRY(hipSetDevice(f->device));
+    const bool bdev = is_device_ptr(b), xdev = is_device_ptr(x);
+    const double *db = b;
+    double *dx = x;
+    int ldb_d = ldb, ldx_d = ldx;
+    const size_t need = (size_t)n * nrhs;
+    if (!bdev || !xdev) {
+        if (f->stage_cap < need) {
+            if (f->d_stage_b) dfree(f, f->d_stage_b, sizeof(double) * f->stage_cap);
+            if (f->d_stage_x) dfree(f, f->d_stage_x, sizeof(double) * f->stage_cap);
+            f->d_stage_b = f->d_stage_x = nullptr;
+            f->stage_cap = 0;
+            int rc;
+            if ((rc = f->dalloc(&f->d_stage_b, need)) || (rc = f->dalloc(&f->d_stage_x, need))) return rc;
+            f->stage_cap = need;
+        }
+    }
+    if (!bdev) {
+        HIP_TRY(hipMemcpy2DAsync(f->d_stage_b, sizeof(double) * n, b, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                 hipMemcpyHostToDevice, s));
+        db = f->d_stage_b;
+        ldb_d = n;
+    }
+    if (!xdev) {
+        dx = f->d_stage_x;
+        ldx_d = n;
+    }
+    hipEvent_t ev[5];
+    if (timed)
+        for (int i = 0; i < 5; i++) HIP_TRY(hipEventCreate(&ev[i]));
+    for (int j = 0; j < nrhs; j++) {
+        int rc = solve_one(f, db + (size_t)j * ldb_d, dx + (size_t)j * ldx_d, s, timed ? ev : nullptr);
+        if (rc) return rc;
+        if (timed) {
+            HIP_TRY(hipEventSynchronize(ev[4]));
+            float ms[4];
+            for (int i = 0; i < 4; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+            g_timers[5] += ms[0] * 1e-3;
+            g_timers[6] += ms[1] * 1e-3;
+            g_timers[7] += ms[2] * 1e-3;
+            g_timers[8] += ms[3] * 1e-3;
+            g_timers[4] += (ms[0] + ms[1] + ms[2] + ms[3]) * 1e-3;
+        }
+    }
+    if (timed)
+        for (int i = 0; i < 5; i++) (void)hipEventDestroy(ev[i]);
+    if (!xdev) {
+        HIP_TRY(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx, sizeof(double) * n, sizeof(double) * n, nrhs,
+                                 hipMemcpyDeviceToHost, s));
+    }
+    if (!bdev || !xdev) HIP_TRY(hipStreamSynchronize(s));
+    return SPIKE_OK;
+}
+
+// ---------------------------------------------------------------- legacy state
+
+std::mutex g_mu;
+spike_factor_t g_cur = nullptr;
+int g_partitions_override = 0;
+
+int legacy_partitions(int n, int nthreads)
+{
+    int P = g_partitions_override;
+    if (P <= 0) {
+        const char *e = std::getenv("SPIKE_PSTRSV_PARTITIONS");
+        if (e && *e) P = std::atoi(e);
+    }
+    if (P <= 0) P = nthreads;
+    if (P <= 0) P = 1;
+    return std::min(P, std::max(n, 1));
+}
+
+void legacy_preproc(double *a, int *ia, int *ja, int n, int nthreads, char uplo, const char *who)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_cur) {
+        spike_factor_destroy(g_cur);
+        g_cur = nullptr;
+    }
+    spike_options_t o;
+    spike_options_default(&o);
+    o.partitions = legacy_partitions(n, nthreads);
+    spike_factor_t f = nullptr;
+    int rc = spike_factor_create(&f, uplo, n, a, ia, ja, &o);
+    if (rc != SPIKE_OK) {
+        std::fprintf(stderr, "%s: %s\n", who, t_err.c_str());
+        return;
+    }
+    g_cur = f;
+}
+
+void legacy_solve(char uplo, int m, double *b, double *x, const char *who, bool timed)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_cur) {
+        std::fprintf(stderr, "%s: no preprocessed matrix (call spike_pstrsv_preproc first)\n", who);
+        return;
+    }
+    char u = (char)(uplo == 'l' ? 'L' : uplo == 'u' ? 'U' : uplo);
+    if (u != g_cur->plan.uplo) {
+        std::fprintf(stderr, "%s: uplo '%c' does not match the preprocessed matrix ('%c')\n", who, uplo,
+                     g_cur->plan.uplo);
+        return;
+    }
+    if (m != g_cur->plan.n) {
+        std::fprintf(stderr, "%s: m = %d does not match the preprocessed matrix (n = %d)\n", who, m, g_cur->plan.n);
+        return;
+    }
+    int rc = factor_solve(g_cur, 1, b, m, x, m, nullptr, timed);
+    if (rc != SPIKE_OK) std::fprintf(stderr, "%s: %s\n", who, t_err.c_str());
+}
+
+}  // namespace
+
+extern "C" {
+
+void spike_options_default(spike_options_t *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->partitions = 0;
+    o->drop_tol = -1.0;
+    o->precision = SPIKE_PREC_FP64;
+    o->lanes_per_row = 0;
+    o->verbose = 0;
+}
+
+int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, const int *ia, const int *ja,
+                        const spike_options_t *opts)
+{
+    if (!out) return fail(SPIKE_ERR_ARG, "factor pointer is NULL");
+    *out = nullptr;
+    if (uplo != 'L' && uplo != 'l' && uplo != 'U' && uplo != 'u') return fail(SPIKE_ERR_ARG, "uplo must be L or U");
+    if (n < 1) return fail(SPIKE_ERR_ARG, "n must be >= 1");
+    if (!a || !ia || !ja) return fail(SPIKE_ERR_ARG, "a, ia and ja must be non-NULL");
+    spike_options_t o;
+    spike_options_default(&o);
+    if (opts) o = *opts;
+    if (o.precision != SPIKE_PREC_FP64) return fail(SPIKE_ERR_UNSUPPORTED, "unsupported precision mode");
+    spike_factor_s *f = new spike_factor_s();
+    f->opts = o;
+    int rc = factor_build(f, uplo, n, a, ia, ja);
+    if (rc != SPIKE_OK) {
+        std::string keep = t_err;
+        f->release();
+        delete f;
+        t_err = keep;
+        return rc;
+    }
+    *out = f;
+    t_err.clear();
+    return SPIKE_OK;
+}
+
+int spike_factor_solve(spike_factor_t f, int nrhs, const double *b, int ldb, double *x, int ldx, void *stream)
+{
+    if (!f) return fail(SPIKE_ERR_STATE, "factor is NULL");
+    return factor_solve(f, nrhs, b, ldb, x, ldx, (hipStream_t)stream, false);
+}
+
+int spike_factor_destroy(spike_factor_t f)
+{
+    if (!f) return SPIKE_OK;
+    f->release();
+    delete f;
+    return SPIKE_OK;
+}
+
+int spike_factor_stats(spike_factor_t f, spike_stats_t *st)
+{
+    if (!f || !st) return fail(SPIKE_ERR_ARG, "NULL argument");
+    *st = f->stats;
+    st->device_bytes = f->device_bytes;
+    return SPIKE_OK;
+}
+
+int spike_factor_log(spike_factor_t f)
+{
+    if (!f) return fail(SPIKE_ERR_ARG, "factor is NULL");
+    const Plan &pl = f->plan;
+    std::printf("* * * * * * * * * * * * * * * * * * * * * *\n");
+    std::printf("* Matrix Information\n");
+    std::printf("* ------------------\n");
+    std::printf("* Reduced system size: %d x %d\n", f->stats.n_reduced, f->stats.n_reduced);
+    std::printf("* Reduced system nnzs: %lld\n", (long long)f->stats.nnz_reduced);
+    for (int t = 0; t < pl.P; t++) {
+        const spike::Part &pt = pl.parts[t];
+        std::printf("* S[%d] size: %d x %d\n", t, pt.rows_S, pt.kcnt);
+        std::printf("* isOptimized[%d]: %d\n", t, 0);
+        std::printf("* hasReflection[%d]: %d\n", t, pt.dcnt > 0 ? 1 : 0);
+        std::printf("* hasDependences[%d]: %d\n", t, pt.ccnt > 0 ? 1 : 0);
+        std::printf("* hasToCalculateS[%d]: %d\n", t, pt.rows_S > 0 && pt.kcnt > 0 ? 1 : 0);
+        std::printf("* Intervals start[%d]: %d, end[%d]: %d\n", t, pt.f, t, pt.g + 1);
+        std::printf("* Partition start[%d]: %d, end[%d]: %d\n", t, pt.s, t, pt.s + pt.k);
+    }
+    std::printf("* * * * * * * * * * * * * * * * * * * * * *\n\n");
+    std::fflush(stdout);
+    return SPIKE_OK;
+}
+
+int spike_analyze(char uplo, int n, const double *a, const int *ia, const int *ja, const spike_options_t *opts,
+                  spike_stats_t *st)
+{
+    if (!st) return fail(SPIKE_ERR_ARG, "stats pointer is NULL");
+    if (uplo != 'L' && uplo != 'l' && uplo != 'U' && uplo != 'u') return fail(SPIKE_ERR_ARG, "uplo must be L or U");
+    if (n < 1) return fail(SPIKE_ERR_ARG, "n must be >= 1");
+    spike_options_t o;
+    spike_options_default(&o);
+    if (opts) o = *opts;
+    spike::PlanOptions po;
+    int P = o.partitions > 0 ? o.partitions : std::max(1, n / 1024);
+    po.P = std::min(P, n);
+    po.lanes_per_row = o.lanes_per_row;
+    Plan pl;
+    std::string err = spike::build_plan(uplo, n, a, ia, ja, po, pl);
+    if (!err.empty()) return fail(SPIKE_ERR_MATRIX, err);
+    std::memset(st, 0, sizeof *st);
+    st->n = n;
+    st->partitions = pl.P;
+    st->nnz = pl.nnz;
+    st->n_reduced = (int)pl.dep_rows.size();
+    st->spike_area = pl.spike_area;
+    st->steps_pass1 = pl.pass1.max_steps;
+    st->steps_pass2 = pl.pass2.max_steps;
+    st->lanes_per_row = pl.pass1.G;
+    return SPIKE_OK;
+}
+
+void spike_pstrsv_set_partitions(int P)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_partitions_override = P > 0 ? P : 0;
+}
+
+spike_factor_t spike_pstrsv_current_factor(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_cur;
+}
+
+const char *spike_last_error(void) { return t_err.c_str(); }
+
+int spike_pstrsv_get_timers(double *t, int cap)
+{
+    int m = std::min(cap, 9);
+    for (int i = 0; i < m; i++) t[i] = g_timers[i];
+    return m;
+}
+
+void spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo)
+{
+    legacy_preproc(a, ia, ja, n, nthreads, uplo, "spike_pstrsv_preproc");
+}
+
+void _spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo)
+{
+    // timers restart per preprocess, like initTimers_PSTRSV3 (ops.c:108-120)
+    for (double &v : g_timers) v = 0.0;
+    legacy_preproc(a, ia, ja, n, nthreads, uplo, "_spike_pstrsv_preproc");
+}
+
+void spike_pstrsv(const char uplo, int m, double *a, MKL_INT *ia, MKL_INT *ja, double *b, double *x, int nthreads)
+{
+    legacy_preproc(a, ia, ja, m, nthreads, uplo, "spike_pstrsv");
+    legacy_solve(uplo, m, b, x, "spike_pstrsv", false);
+}
+
+void spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthreads)
+{
+    (void)nthreads;
+    legacy_solve(uplo, m, b, x, "spike_pstrsv_solve", false);
+}
+
+void _spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthreads)
+{
+    (void)nthreads;
+    legacy_solve(uplo, m, b, x, "_spike_pstrsv_solve", true);
+}
+
+}  // extern "C"

@@ -1,0 +1,433 @@
+// gfx950 kernels of the Spike sparse triangular solve (FP64).
+//
+// Block solve (pass 1 / pass 2): one 64-lane wavefront per partition.  The
+// partition's x segment lives in LDS (k+1 doubles; slot k holds 0.0 and is the
+// target of padding entries).  Before the steps run, xs[r] holds the row's
+// right-hand side; a step slot then computes
+//     xs[row] = (xs[row] - sum_e val_e * xs[col_e]) * mult
+// where mult = 1/diag for a row's final chunk and 1 for a partial chunk of a
+// long row.  Steps execute in order and each step only reads rows written by
+// earlier steps; LDS operations of one wavefront complete in program order, so
+// no flags or barriers are needed between steps.  Step records are streamed
+// from HBM through a register ring D steps deep, so the latency of the loads
+// is off the dependency chain.  The G lanes of a row slot are reduced with
+// DPP (quad_perm / row_half_mirror / row_mirror), in a fixed order, so results
+// are bit-for-bit reproducible.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace spike {
+
+namespace {
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    int lo = (int)(unsigned)(u & 0xffffffffull);
+    int hi = (int)(unsigned)(u >> 32);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over aligned groups of G lanes; every lane of a group gets the same bits.
+template <int G>
+__device__ __forceinline__ double group_sum(double v)
+{
+    if constexpr (G >= 2) v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    if constexpr (G >= 4) v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror
+    if constexpr (G >= 16) v += dpp_f64<0x140>(v); // row_mirror
+    return v;
+}
+
+template <int G, int M, int D, int PASS>
+__global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
+{
+    extern __shared__ double xs[];
+    constexpr int R = 64 / G;
+    const int p = blockIdx.x;
+    const DevPart pt = A.parts[p];
+    const int lane = threadIdx.x;
+    const int slot = lane / G, li = lane % G;
+    const int s = pt.s, k = pt.k;
+    int lo, hi;
+    if (PASS == 1) {
+        if (pt.hi1 < 0) return;
+        lo = 0;
+        hi = pt.hi1;
+    } else {
+        if (pt.f >= k) return;
+        lo = pt.f;
+        hi = k - 1;
+    }
+    const int n = A.n, flip = A.flip;
+
+    // right-hand side (pass 2: rows before f are final from pass 1)
+    if (PASS == 1) {
+        for (int r = lane; r <= hi; r += 64) {
+            int g = s + r;
+            xs[r] = A.b[flip ? n - 1 - g : g];
+        }
+    } else {
+        for (int r = lane; r < k; r += 64) {
+            int g = s + r, m = flip ? n - 1 - g : g;
+            xs[r] = r < lo ? A.x[m] : A.b[m];
+        }
+    }
+    if (lane == 0) xs[k] = 0.0;
+    if (PASS == 2) {
+        __syncthreads();
+        for (int c = lane; c < pt.ccnt; c += 64) xs[A.cpl_rows[pt.cbeg + c] - s] = A.bc[pt.cbeg + c];
+    }
+    __syncthreads();
+
+    const int64_t sb = A.st.part_begin[p];
+    const int nst = (int)(A.st.part_begin[p + 1] - sb);
+    const double *__restrict__ sval = A.st.val;
+    const uint16_t *__restrict__ scol = A.st.col;
+    const uint16_t *__restrict__ srow = A.st.row;
+    const double *__restrict__ smul = A.st.mult;
+
+    // Register ring of D step records.  It starts as empty padding steps
+    // (column k, no row) and the loop starts D steps early, so the first D
+    // iterations compute nothing and only issue loads.  Every load then comes
+    // from the same instructions in the same order, and the loop runs whole
+    // iterations only (the last < D steps run in the unrolled tail below), so
+    // the compiler's memory wait counters stay exact across iterations and
+    // loads stay D steps in flight.
+    double rv[D][M];
+    unsigned rc[D][M];
+    unsigned rrow[D];
+    double rmul[D];
+    if (nst <= 0) goto epilogue;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            rv[d][m] = 0.0;
+            rc[d][m] = (unsigned)k;
+        }
+        rrow[d] = 0xFFFFu;
+        rmul[d] = 0.0;
+    }
+    {
+        // One step from ring slot d: LDS reads first (operands and the row's
+        // accumulator), then the dot product with the accumulator folded into
+        // lane 0 of the group, so the accumulator read stays ahead of the
+        // reduction.
+        auto step = [&](int d) __attribute__((always_inline)) {
+            const unsigned row = rrow[d];
+            const bool valid = row != 0xFFFFu;
+            double xv[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) xv[m] = xs[rc[d][m]];
+            const double bv = xs[valid ? row : (unsigned)k];
+            double t = 0.0;
+#pragma unroll
+            for (int m = 0; m < M; m++) t = fma(-rv[d][m], xv[m], t);
+            t = li == 0 ? t + bv : t;
+            t = group_sum<G>(t);
+            const double res = t * rmul[d];
+            if (li == 0 && valid) xs[row] = res;
+        };
+        // Refill slot d with step sidx + D (clamped to the last step), after
+        // the last use of the slot's registers.
+        auto refill = [&](int d, int sidx) __attribute__((always_inline)) {
+            const int nxt = sidx + D < nst ? sidx + D : nst - 1;
+            const int64_t st = sb + nxt;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const size_t idx = ((size_t)st * M + m) * 64 + lane;
+                rv[d][m] = sval[idx];
+                rc[d][m] = scol[idx];
+            }
+            rrow[d] = srow[st * R + slot];
+            rmul[d] = smul[st * R + slot];
+        };
+        int base = -D;
+        for (; base + D <= nst; base += D) {
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                step(d);
+                refill(d, base + d);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++)
+            if (base + d < nst) step(d);
+    }
+epilogue:
+    __syncthreads();
+
+    if (PASS == 1) {
+        for (int q = lane; q < pt.dcnt; q += 64) A.xred[pt.dbeg + q] = xs[A.dep_rows[pt.dbeg + q] - s];
+        const int fin = pt.f < hi + 1 ? pt.f : hi + 1;   // rows before f are final
+        for (int r = lane; r < fin; r += 64) {
+            int g = s + r;
+            A.x[flip ? n - 1 - g : g] = xs[r];
+        }
+    } else {
+        for (int r = lo + lane; r < k; r += 64) {
+            int g = s + r;
+            A.x[flip ? n - 1 - g : g] = xs[r];
+        }
+    }
+}
+
+template <int G, int PASS>
+hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
+{
+    constexpr int M = 1, D = 8;
+    size_t lds = (size_t)(a.kmax + 1) * sizeof(double);
+    auto kern = k_block_solve<G, M, D, PASS>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(P), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int PASS>
+hipError_t launch_pass(int P, const BlockArgs &a, hipStream_t s)
+{
+    switch (a.st.G) {
+    case 1: return launch_g<1, PASS>(P, a, s);
+    case 2: return launch_g<2, PASS>(P, a, s);
+    case 4: return launch_g<4, PASS>(P, a, s);
+    case 8: return launch_g<8, PASS>(P, a, s);
+    case 16: return launch_g<16, PASS>(P, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------- reduced
+
+template <bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, const int *__restrict__ levptr,
+                                                        const int *__restrict__ rlist, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci, const double *__restrict__ v,
+                                                        double *xred)
+{
+    extern __shared__ double xl[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *X = xred;
+    if (IN_LDS) {
+        for (int i = tid; i < nred; i += nt) xl[i] = xred[i];
+        __syncthreads();
+        X = xl;
+    }
+    for (int L = 0; L < nlev; L++) {
+        const int b = levptr[L], e = levptr[L + 1];
+        for (int idx = b + tid; idx < e; idx += nt) {
+            const int r = rlist[idx];
+            double acc = 0.0;
+            for (int j = rp[r]; j < rp[r + 1]; j++) acc = fma(v[j], X[ci[j]], acc);
+            X[r] -= acc;
+        }
+        __syncthreads();
+    }
+    if (IN_LDS)
+        for (int i = tid; i < nred; i += nt) xred[i] = xl[i];
+}
+
+// ---------------------------------------------------------------- coupling
+
+__global__ __launch_bounds__(256) void k_coupling(int ncpl, const int *__restrict__ rows,
+                                                  const int *__restrict__ ptr, const int *__restrict__ col,
+                                                  const double *__restrict__ val, const double *__restrict__ xred,
+                                                  const double *__restrict__ b, double *__restrict__ bc, int n,
+                                                  int flip)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncpl) return;
+    const int g = rows[c];
+    double acc = 0.0;
+    for (int j = ptr[c]; j < ptr[c + 1]; j++) acc = fma(val[j], xred[col[j]], acc);
+    bc[c] = b[flip ? n - 1 - g : g] - acc;
+}
+
+// ---------------------------------------------------------------- spikes
+
+__global__ __launch_bounds__(256) void k_spike_build(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
+                                                     const int *__restrict__ ia, const int *__restrict__ ja,
+                                                     const double *__restrict__ a, double *W)
+{
+    const SpikePart sp = sps[blockIdx.x];
+    for (int j = threadIdx.x; j < sp.kcnt; j += blockDim.x) {
+        const int myk = kcols[sp.kbeg + j];
+        double *Wc = W + sp.woff + j;
+        const int base = sp.s + sp.f;
+        for (int r = 0; r < sp.rows; r++) {
+            const int i = base + r;
+            const int e0 = ia[i], e1 = ia[i + 1] - 1;   // e1: diagonal
+            double w = 0.0;
+            int e = e0;
+            for (; e < e1 && ja[e] < sp.s; e++)
+                if (ja[e] == myk) w = a[e];
+            // in-block entries: only spike rows (>= base) are nonzero
+            for (; e + 4 <= e1; e += 4) {
+                const int c0 = ja[e] - base, c1 = ja[e + 1] - base, c2 = ja[e + 2] - base, c3 = ja[e + 3] - base;
+                const double w0 = c0 >= 0 ? Wc[(size_t)c0 * sp.kcnt] : 0.0;
+                const double w1 = c1 >= 0 ? Wc[(size_t)c1 * sp.kcnt] : 0.0;
+                const double w2 = c2 >= 0 ? Wc[(size_t)c2 * sp.kcnt] : 0.0;
+                const double w3 = c3 >= 0 ? Wc[(size_t)c3 * sp.kcnt] : 0.0;
+                w -= a[e] * w0;
+                w -= a[e + 1] * w1;
+                w -= a[e + 2] * w2;
+                w -= a[e + 3] * w3;
+            }
+            for (; e < e1; e++) {
+                const int c = ja[e] - base;
+                if (c >= 0) w -= a[e] * Wc[(size_t)c * sp.kcnt];
+            }
+            Wc[(size_t)r * sp.kcnt] = w / a[e1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- assembly
+
+__device__ __forceinline__ bool keep_entry(double w, double tol)
+{
+    return tol < 0.0 ? (w != 0.0) : (fabs(w) > tol);
+}
+
+__global__ __launch_bounds__(256) void k_reduced_count(int nred, const int64_t *__restrict__ wrow,
+                                                       const int *__restrict__ kcnt, const double *__restrict__ tol,
+                                                       const double *__restrict__ W, int *cnt, double *dropped)
+{
+    const int q = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (q >= nred) return;
+    const int64_t w0 = wrow[q];
+    int c = 0;
+    double d = 0.0;
+    if (w0 >= 0) {
+        const int m = kcnt[q];
+        const double t = tol[q];
+        for (int l = lane; l < m; l += 64) {
+            const double w = W[w0 + l];
+            if (keep_entry(w, t)) c++;
+            else d += fabs(w);
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        c += __shfl_xor(c, off);
+        d += __shfl_xor(d, off);
+    }
+    if (lane == 0) {
+        cnt[q] = c;
+        dropped[q] = d;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduced_fill(int nred, const int64_t *__restrict__ wrow,
+                                                      const int *__restrict__ kbeg, const int *__restrict__ kcnt,
+                                                      const double *__restrict__ tol, const double *__restrict__ W,
+                                                      const int *__restrict__ kridx, const int *__restrict__ rp,
+                                                      int *ci, double *v)
+{
+    const int q = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (q >= nred) return;
+    const int64_t w0 = wrow[q];
+    if (w0 < 0) return;
+    const int m = kcnt[q], kb = kbeg[q];
+    const double t = tol[q];
+    int pos = rp[q];
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int l0 = 0; l0 < m; l0 += 64) {
+        const int l = l0 + lane;
+        double w = 0.0;
+        bool keep = false;
+        if (l < m) {
+            w = W[w0 + l];
+            keep = keep_entry(w, t);
+        }
+        const unsigned long long mask = __ballot(keep);
+        if (keep) {
+            const int at = pos + __popcll(mask & below);
+            ci[at] = kridx[kb + l];
+            v[at] = w;
+        }
+        pos += __popcll(mask);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s)
+{
+    if (P <= 0) return hipSuccess;
+    return pass == 1 ? launch_pass<1>(P, a, s) : launch_pass<2>(P, a, s);
+}
+
+hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int *rlist, const int *rp,
+                                const int *ci, const double *v, double *xred, bool in_lds, hipStream_t s)
+{
+    if (nlev <= 0) return hipSuccess;
+    if (in_lds) {
+        size_t lds = (size_t)nred * sizeof(double);
+        auto kern = k_reduced_solve<true>;
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(kern, dim3(1), dim3(1024), lds, s, nred, nlev, levptr, rlist, rp, ci, v, xred);
+    } else {
+        hipLaunchKernelGGL(k_reduced_solve<false>, dim3(1), dim3(1024), 0, s, nred, nlev, levptr, rlist, rp, ci, v,
+                           xred);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
+                           const double *cpl_val, const double *xred, const double *b, double *bc, int n, int flip,
+                           hipStream_t s)
+{
+    if (ncpl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_coupling, dim3((ncpl + 255) / 256), dim3(256), 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col,
+                       cpl_val, xred, b, bc, n, flip);
+    return hipGetLastError();
+}
+
+hipError_t launch_spike_build(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,
+                              const double *a, double *W, hipStream_t s)
+{
+    if (nsp <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spike_build, dim3(nsp), dim3(256), 0, s, sp, kcols, ia, ja, a, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduced_count(int nred, const int64_t *wrow, const int *kcnt, const double *tol, const double *W,
+                                int *cnt, double *dropped, hipStream_t s)
+{
+    if (nred <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduced_count, dim3((nred + 3) / 4), dim3(256), 0, s, nred, wrow, kcnt, tol, W, cnt,
+                       dropped);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduced_fill(int nred, const int64_t *wrow, const int *kbeg, const int *kcnt, const double *tol,
+                               const double *W, const int *kridx, const int *rp, int *ci, double *v, hipStream_t s)
+{
+    if (nred <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduced_fill, dim3((nred + 3) / 4), dim3(256), 0, s, nred, wrow, kbeg, kcnt, tol, W, kridx,
+                       rp, ci, v);
+    return hipGetLastError();
+}
+
+size_t block_solve_lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 65536;
+    return (size_t)v;
+}
+
+}  // namespace spike

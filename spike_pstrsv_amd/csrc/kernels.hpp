@@ -1,0 +1,80 @@
+// Launch wrappers of the gfx950 kernels (implemented in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+
+namespace spike {
+
+// Per-partition record read by the block-solve kernels.
+struct DevPart {
+    int s, k;       // rows [s, s+k) of the lower-triangular form
+    int f;          // first coupling row (local), k if none
+    int hi1;        // pass 1 solves local rows [0, hi1]; -1 = nothing
+    int dbeg, dcnt; // dependents (reduced indices dbeg .. dbeg+dcnt)
+    int cbeg, ccnt; // coupling rows (indices into the coupling arrays)
+};
+
+// Device view of a StepStream.
+struct DevSteps {
+    const double *val;
+    const uint16_t *col;
+    const uint16_t *row;
+    const double *mult;
+    const int64_t *part_begin;   // P+1
+    int G, M, R;
+};
+
+struct BlockArgs {
+    const DevPart *parts;
+    DevSteps st;
+    const double *b;        // right-hand side (caller's index order)
+    double *x;              // solution (caller's index order)
+    double *xred;           // reduced vector (pass 1 writes y at dependents)
+    const double *bc;       // pass 2: b - C x_K at coupling rows
+    const int *dep_rows;    // global (lower-form) rows of the dependents
+    const int *cpl_rows;    // global (lower-form) rows of the coupling rows
+    int n;
+    int flip;               // 1: caller's index = n-1-row
+    int kmax;               // largest partition (LDS x segment = kmax+1 doubles)
+};
+
+// Pass 1 (y = D^-1 b on rows [0, hi1]) or pass 2 (re-solve rows [f, k) with
+// the coupling-corrected right-hand side), one wavefront per partition.
+hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s);
+
+// Single-workgroup forward solve of the unit lower reduced system, in place on
+// xred.  rlist holds the rows with off-diagonals, grouped by level;
+// levptr[nlev+1] (device) delimits the levels.
+hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int *rlist,
+                                const int *rp, const int *ci, const double *v,
+                                double *xred, bool in_lds, hipStream_t s);
+
+// bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c.
+hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
+                           const double *cpl_val, const double *xred, const double *b,
+                           double *bc, int n, int flip, hipStream_t s);
+
+// Spike build: for every partition with kcnt > 0, W_p = D0^-1 C[:, K_p] on
+// rows [f, g] (row-major, ld = kcnt) using the lower-form CSR (ia, ja, a).
+struct SpikePart {
+    int s, f, rows, kbeg, kcnt;
+    int64_t woff;
+};
+hipError_t launch_spike_build(int nsp, const SpikePart *sp, const int *kcols, const int *ia,
+                              const int *ja, const double *a, double *W, hipStream_t s);
+
+// Reduced-system assembly.  For reduced row q: its spike row is
+// W[wrow[q] .. wrow[q]+kcnt[q]) (wrow < 0: identity row); entry l is kept when
+// |w| > tol[q] (tol < 0: keep w != 0).  kridx[kbeg[q] + l] is the reduced
+// column.  count pass: cnt[q], dropped[q] = sum of dropped |w|.
+hipError_t launch_reduced_count(int nred, const int64_t *wrow, const int *kcnt, const double *tol,
+                                const double *W, int *cnt, double *dropped, hipStream_t s);
+hipError_t launch_reduced_fill(int nred, const int64_t *wrow, const int *kbeg, const int *kcnt,
+                               const double *tol, const double *W, const int *kridx,
+                               const int *rp, int *ci, double *v, hipStream_t s);
+
+// Largest dynamic LDS a block-solve workgroup may use, in bytes.
+size_t block_solve_lds_limit();
+
+}  // namespace spike

@@ -1,0 +1,372 @@
+// Host-side symbolic analysis for the Spike solve.  See plan.hpp.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <thread>
+
+namespace spike {
+
+namespace {
+
+template <class F>
+void parallel_for(int count, F &&fn)
+{
+    int nt = (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > 32) nt = 32;
+    if (count < 2 * nt) nt = std::max(1, std::min(nt, count));
+    if (nt == 1) {
+        for (int i = 0; i < count; i++) fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&]() {
+            for (;;) {
+                int i = next.fetch_add(1);
+                if (i >= count) break;
+                fn(i);
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
+std::string validate(char uplo, int n, const int *ia, const int *ja)
+{
+    if (ia[0] != 0) return "ia[0] must be 0";
+    for (int i = 0; i < n; i++) {
+        if (ia[i + 1] <= ia[i]) return "row " + std::to_string(i) + " is empty (diagonal missing)";
+        for (int j = ia[i]; j < ia[i + 1]; j++) {
+            if (ja[j] < 0 || ja[j] >= n) return "column index out of range in row " + std::to_string(i);
+            if (j > ia[i] && ja[j] <= ja[j - 1])
+                return "columns of row " + std::to_string(i) + " are not strictly ascending";
+        }
+        if (uplo == 'L') {
+            if (ja[ia[i + 1] - 1] != i) return "row " + std::to_string(i) + ": diagonal must be the last entry ('L')";
+        } else {
+            if (ja[ia[i]] != i) return "row " + std::to_string(i) + ": diagonal must be the first entry ('U')";
+        }
+    }
+    return "";
+}
+
+// A schedulable unit: one chunk of one row.
+struct Task {
+    int row;        // local row
+    int eb, ee;     // entry range in the partition-local entry list
+    int level;
+    bool final_chunk;
+};
+
+// Builds the steps of one partition over local rows [lo, hi] (inclusive).
+// Entries of row r: local columns ecol[rb[r]..rb[r+1]) (ascending, < r).
+void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
+                        const std::vector<int> &ecol, const std::vector<double> &evals,
+                        const std::vector<double> &diag, int G, int M,
+                        std::vector<double> &val, std::vector<uint16_t> &col,
+                        std::vector<uint16_t> &row, std::vector<double> &mult, int &nsteps)
+{
+    const int R = 64 / G, cap = G * M;
+    std::vector<Task> tasks;
+    std::vector<int> final_task(k, -1);      // index of the final task of a row
+    std::vector<int> task_level;
+    for (int r = lo; r <= hi; r++) {
+        int len = rb[r + 1] - rb[r];
+        int nch = len == 0 ? 1 : (len + cap - 1) / cap;
+        int prev_level = -1;
+        for (int c = 0; c < nch; c++) {
+            Task t;
+            t.row = r;
+            t.eb = rb[r] + c * cap;
+            t.ee = std::min(rb[r + 1], t.eb + cap);
+            int lev = prev_level + 1;  // after the previous chunk of the row
+            for (int e = t.eb; e < t.ee; e++) {
+                int d = ecol[e];
+                if (d >= lo) lev = std::max(lev, tasks[final_task[d]].level + 1);
+            }
+            t.level = lev;
+            t.final_chunk = (c == nch - 1);
+            prev_level = lev;
+            tasks.push_back(t);
+        }
+        final_task[r] = (int)tasks.size() - 1;
+    }
+    const int ntask = (int)tasks.size();
+    std::vector<int> order(ntask);
+    for (int i = 0; i < ntask; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return tasks[x].level < tasks[y].level;
+    });
+    // greedy packing: a task may not share a step with a task it depends on
+    std::vector<int> step_of(ntask, -1);
+    std::vector<int> prev_chunk(ntask, -1);
+    for (int i = 1; i < ntask; i++)
+        if (tasks[i].row == tasks[i - 1].row) prev_chunk[i] = i - 1;
+    std::vector<std::vector<int>> steps;
+    int cur = -1, used = R;
+    for (int oi = 0; oi < ntask; oi++) {
+        int ti = order[oi];
+        const Task &t = tasks[ti];
+        bool conflict = used >= R;
+        if (!conflict && prev_chunk[ti] >= 0 && step_of[prev_chunk[ti]] == cur) conflict = true;
+        if (!conflict)
+            for (int e = t.eb; e < t.ee; e++) {
+                int d = ecol[e];
+                if (d >= lo && step_of[final_task[d]] == cur) { conflict = true; break; }
+            }
+        if (conflict) {
+            steps.emplace_back();
+            cur = (int)steps.size() - 1;
+            used = 0;
+        }
+        steps[cur].push_back(ti);
+        step_of[ti] = cur;
+        used++;
+    }
+    nsteps = (int)steps.size();
+    val.assign((size_t)nsteps * M * 64, 0.0);
+    col.assign((size_t)nsteps * M * 64, (uint16_t)k);
+    row.assign((size_t)nsteps * R, (uint16_t)0xFFFF);
+    mult.assign((size_t)nsteps * R, 0.0);
+    for (int s = 0; s < nsteps; s++) {
+        for (int slot = 0; slot < (int)steps[s].size(); slot++) {
+            const Task &t = tasks[steps[s][slot]];
+            row[(size_t)s * R + slot] = (uint16_t)t.row;
+            mult[(size_t)s * R + slot] = t.final_chunk ? 1.0 / diag[t.row] : 1.0;
+            for (int e = t.eb; e < t.ee; e++) {
+                int q = e - t.eb;
+                int m = q / G, li = q % G;
+                size_t idx = ((size_t)s * M + m) * 64 + slot * G + li;
+                val[idx] = evals[e];
+                col[idx] = (uint16_t)ecol[e];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+std::vector<int> row_levels(int m, const int *rp, const int *ci)
+{
+    std::vector<int> lev(m, 0);
+    for (int i = 0; i < m; i++) {
+        int l = 0;
+        for (int j = rp[i]; j < rp[i + 1]; j++) l = std::max(l, lev[ci[j]] + 1);
+        lev[i] = l;
+    }
+    return lev;
+}
+
+int choose_lanes(const std::vector<int> &row_len)
+{
+    if (row_len.empty()) return 1;
+    std::vector<int> v(row_len);
+    size_t q = (v.size() * 9) / 10;
+    std::nth_element(v.begin(), v.begin() + q, v.end());
+    int p90 = v[q];
+    int G = 1;
+    while (G < p90 && G < 16) G <<= 1;
+    return G;
+}
+
+std::string build_plan(char uplo, int n, const double *a, const int *ia, const int *ja,
+                       const PlanOptions &opt, Plan &pl)
+{
+    if (uplo == 'l') uplo = 'L';
+    if (uplo == 'u') uplo = 'U';
+    if (uplo != 'L' && uplo != 'U') return "uplo must be 'L' or 'U'";
+    if (n < 1) return "n must be >= 1";
+    if (!a || !ia || !ja) return "a, ia and ja must be non-NULL";
+    int P = opt.P;
+    if (P < 1 || P > n) return "partition count must be in [1, n]";
+    std::string err = validate(uplo, n, ia, ja);
+    if (!err.empty()) return err;
+    const int64_t nnz = ia[n];
+    if (nnz > INT32_MAX) return "nnz exceeds 2^31-1";
+
+    pl = Plan();
+    pl.uplo = uplo;
+    pl.flip = (uplo == 'U');
+    pl.n = n;
+    pl.P = P;
+    pl.nnz = nnz;
+    pl.ia.resize(n + 1);
+    pl.ja.resize(nnz);
+    pl.a.resize(nnz);
+    if (!pl.flip) {
+        std::copy(ia, ia + n + 1, pl.ia.begin());
+        std::copy(ja, ja + nnz, pl.ja.begin());
+        std::copy(a, a + nnz, pl.a.begin());
+    } else {
+        // U = J L J: row i of L is row n-1-i of U reversed, columns mirrored
+        pl.ia[0] = 0;
+        for (int i = 0; i < n; i++) {
+            int src = n - 1 - i;
+            pl.ia[i + 1] = pl.ia[i] + (ia[src + 1] - ia[src]);
+        }
+        parallel_for(n, [&](int i) {
+            int src = n - 1 - i;
+            int p = pl.ia[i];
+            for (int j = ia[src + 1] - 1; j >= ia[src]; j--, p++) {
+                pl.a[p] = a[j];
+                pl.ja[p] = n - 1 - ja[j];
+            }
+        });
+    }
+    const int *LA = pl.ia.data();
+    const int *LJ = pl.ja.data();
+    const double *LV = pl.a.data();
+
+    // partitions (matrix_partitioning.c:18-33): equal blocks, remainder last
+    pl.starts.resize(P + 1);
+    int kb = n / P;
+    for (int t = 0; t < P; t++) pl.starts[t] = kb * t;
+    pl.starts[P] = n;
+    pl.kmax = n - kb * (P - 1);
+    if (pl.kmax > opt.kmax_limit)
+        return "partition of " + std::to_string(pl.kmax) + " rows exceeds the device limit of " +
+               std::to_string(opt.kmax_limit) + " rows; use more partitions";
+
+    // dependents (ops.c:162-181, lower branch)
+    std::vector<char> isdep(n, 0);
+    pl.parts.resize(P);
+    for (int t = 0; t < P; t++) {
+        Part &pt = pl.parts[t];
+        pt.s = pl.starts[t];
+        pt.k = pl.starts[t + 1] - pt.s;
+    }
+    for (int i = 0; i < n; i++) {
+        // coupling entries come first in a row (ascending columns)
+        int t = std::min(i / std::max(kb, 1), P - 1);
+        int s = pl.starts[t];
+        for (int j = LA[i]; j < LA[i + 1] && LJ[j] < s; j++) isdep[LJ[j]] = 1;
+    }
+    pl.ridx.assign(n, -1);
+    for (int i = 0; i < n; i++)
+        if (isdep[i]) {
+            pl.ridx[i] = (int)pl.dep_rows.size();
+            pl.dep_rows.push_back(i);
+        }
+
+    // intervals, K_p, dependents and coupling rows per partition
+    std::vector<std::vector<int>> kc(P);
+    parallel_for(P, [&](int t) {
+        Part &pt = pl.parts[t];
+        int s = pt.s, k = pt.k;
+        pt.f = k;
+        pt.g = -1;
+        for (int r = 0; r < k; r++) {
+            int i = s + r;
+            if (pt.f == k && LJ[LA[i]] < s) pt.f = r;
+            if (isdep[i]) pt.g = r;
+        }
+        pt.hi1 = (pt.f == k) ? k - 1 : std::max(pt.g, pt.f - 1);
+        if (pt.hi1 > k - 1) pt.hi1 = k - 1;
+        std::vector<int> &K = kc[t];
+        for (int r = pt.f; r <= pt.g; r++) {
+            int i = s + r;
+            for (int j = LA[i]; j < LA[i + 1] && LJ[j] < s; j++) K.push_back(LJ[j]);
+        }
+        std::sort(K.begin(), K.end());
+        K.erase(std::unique(K.begin(), K.end()), K.end());
+        pt.kcnt = (int)K.size();
+        pt.rows_S = pt.kcnt > 0 ? pt.g - pt.f + 1 : 0;
+    });
+    {
+        int acc = 0;
+        for (int t = 0; t < P; t++) {
+            pl.parts[t].kbeg = acc;
+            acc += pl.parts[t].kcnt;
+            pl.spike_area += (int64_t)pl.parts[t].rows_S * pl.parts[t].kcnt;
+        }
+        pl.kcols.resize(acc);
+        for (int t = 0; t < P; t++) std::copy(kc[t].begin(), kc[t].end(), pl.kcols.begin() + pl.parts[t].kbeg);
+        // dependents are ascending, so each partition's are contiguous
+        size_t d = 0;
+        for (int t = 0; t < P; t++) {
+            Part &pt = pl.parts[t];
+            pt.dbeg = (int)d;
+            while (d < pl.dep_rows.size() && pl.dep_rows[d] < pt.s + pt.k) d++;
+            pt.dcnt = (int)d - pt.dbeg;
+        }
+        // coupling rows (b' = b - C x_K in pass 2)
+        pl.cpl_ptr.push_back(0);
+        for (int t = 0; t < P; t++) {
+            Part &pt = pl.parts[t];
+            pt.cbeg = (int)pl.cpl_rows.size();
+            for (int r = pt.f; r < pt.k; r++) {
+                int i = pt.s + r;
+                if (LJ[LA[i]] >= pt.s) continue;
+                pl.cpl_rows.push_back(i);
+                for (int j = LA[i]; j < LA[i + 1] && LJ[j] < pt.s; j++) {
+                    pl.cpl_col.push_back(pl.ridx[LJ[j]]);
+                    pl.cpl_val.push_back(LV[j]);
+                }
+                pl.cpl_ptr.push_back((int)pl.cpl_col.size());
+            }
+            pt.ccnt = (int)pl.cpl_rows.size() - pt.cbeg;
+        }
+    }
+
+    // block-solve step streams
+    std::vector<int> inblock_len(n);
+    for (int t = 0; t < P; t++) {
+        const Part &pt = pl.parts[t];
+        for (int r = 0; r < pt.k; r++) {
+            int i = pt.s + r, c = 0;
+            for (int j = LA[i]; j < LA[i + 1] - 1; j++) c += (LJ[j] >= pt.s);
+            inblock_len[i] = c;
+        }
+    }
+    int G = opt.lanes_per_row > 0 ? opt.lanes_per_row : choose_lanes(inblock_len);
+    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) return "lanes_per_row must be 1, 2, 4, 8 or 16";
+    const int M = 1;
+    for (int pass = 1; pass <= 2; pass++) {
+        StepStream &ss = pass == 1 ? pl.pass1 : pl.pass2;
+        ss.G = G; ss.M = M; ss.R = 64 / G;
+        std::vector<std::vector<double>> pv(P), pm(P);
+        std::vector<std::vector<uint16_t>> pc(P), pr(P);
+        std::vector<int> pn(P, 0);
+        parallel_for(P, [&](int t) {
+            const Part &pt = pl.parts[t];
+            int lo, hi;
+            if (pass == 1) { lo = 0; hi = pt.hi1; }
+            else { lo = pt.f; hi = pt.k - 1; }
+            if (hi < lo) return;
+            std::vector<int> rb(pt.k + 1, 0), ecol;
+            std::vector<double> ev, dg(pt.k);
+            for (int r = 0; r < pt.k; r++) {
+                int i = pt.s + r;
+                if (r >= lo && r <= hi)
+                    for (int j = LA[i]; j < LA[i + 1] - 1; j++)
+                        if (LJ[j] >= pt.s) { ecol.push_back(LJ[j] - pt.s); ev.push_back(LV[j]); }
+                rb[r + 1] = (int)ecol.size();
+                dg[r] = LV[LA[i + 1] - 1];
+            }
+            schedule_partition(pt.k, lo, hi, rb, ecol, ev, dg, G, M, pv[t], pc[t], pr[t], pm[t], pn[t]);
+        });
+        ss.part_begin.assign(P + 1, 0);
+        for (int t = 0; t < P; t++) {
+            ss.part_begin[t + 1] = ss.part_begin[t] + pn[t];
+            ss.max_steps = std::max(ss.max_steps, pn[t]);
+        }
+        ss.nsteps = ss.part_begin[P];
+        ss.val.resize((size_t)ss.nsteps * M * 64);
+        ss.col.resize((size_t)ss.nsteps * M * 64);
+        ss.row.resize((size_t)ss.nsteps * ss.R);
+        ss.mult.resize((size_t)ss.nsteps * ss.R);
+        parallel_for(P, [&](int t) {
+            size_t s0 = (size_t)ss.part_begin[t];
+            std::copy(pv[t].begin(), pv[t].end(), ss.val.begin() + s0 * M * 64);
+            std::copy(pc[t].begin(), pc[t].end(), ss.col.begin() + s0 * M * 64);
+            std::copy(pr[t].begin(), pr[t].end(), ss.row.begin() + s0 * ss.R);
+            std::copy(pm[t].begin(), pm[t].end(), ss.mult.begin() + s0 * ss.R);
+        });
+    }
+    return "";
+}
+
+}  // namespace spike

@@ -1,0 +1,95 @@
+// Host-side symbolic analysis for the Spike solve (no device code).
+//
+// Everything is expressed for a LOWER triangular matrix: an upper triangular
+// input is converted once by index reversal U = J L J (row i of L is row n-1-i
+// of U, columns c -> n-1-c, entry order reversed), and the device kernels map
+// b/x indices back on load/store.  The Spike quantities follow the reference's
+// construction for 'U' (src/spike_pstrsv_ops.c:129-373, :1071-1150) mirrored
+// to 'L':
+//   partition p      rows [s_p, s_p + k_p)  (src/matrix_partitioning.c:18-33)
+//   coupling C_p     entries of partition p with column < s_p
+//   dependents       columns referenced by any coupling entry (ops.c:162-181)
+//   f_p              first local row holding a coupling entry (k_p if none)
+//   g_p              last local row that is a dependent (-1 if none)
+//   K_p              distinct coupling columns of rows [f_p, g_p]
+//   spike W_p        D_p^-1 C_p[:, K_p] restricted to rows [f_p, g_p]
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace spike {
+
+// One pass of block solves laid out as fixed-shape "steps" executed by one
+// 64-lane wavefront per partition.  A step holds R = 64/G row slots; slot t is
+// served by lanes [t*G, (t+1)*G) and each lane holds M entries ("rounds").
+// Within a partition, steps run in order and every row (chunk) in a step only
+// depends on rows finished in earlier steps.  Rows longer than G*M entries are
+// split into chunks; a non-final chunk subtracts its partial dot product from
+// the row's accumulator and only the final chunk scales by 1/diag.
+struct StepStream {
+    int G = 16, M = 1, R = 4;
+    int64_t nsteps = 0;
+    int max_steps = 0;                  // longest per-partition step chain
+    std::vector<int64_t> part_begin;    // P+1 offsets into the step arrays
+    std::vector<double> val;            // nsteps * M * 64  [step][round][lane]
+    std::vector<uint16_t> col;          // nsteps * M * 64  local column, pad = k_p
+    std::vector<uint16_t> row;          // nsteps * R       local row, 0xFFFF = empty
+    std::vector<double> mult;           // nsteps * R       1/diag (final) or 1 (chunk)
+};
+
+struct Part {
+    int s = 0, k = 0;       // rows [s, s+k)
+    int f = 0;              // first coupling row (local), k if none
+    int g = -1;             // last dependent row (local), -1 if none
+    int hi1 = -1;           // pass 1 solves rows [0, hi1]
+    int kbeg = 0, kcnt = 0; // K_p = kcols[kbeg .. kbeg+kcnt)
+    int rows_S = 0;         // spike rows [f, g] when kcnt > 0
+    int dbeg = 0, dcnt = 0; // dependents of this partition: dep_rows[dbeg..)
+    int cbeg = 0, ccnt = 0; // coupling rows of this partition: cpl_rows[cbeg..)
+};
+
+struct Plan {
+    char uplo = 'L';
+    bool flip = false;          // input was 'U'
+    int n = 0, P = 0;
+    int64_t nnz = 0;
+    int kmax = 0;               // largest partition
+    // lower-triangular CSR (after the flip for 'U'); diagonal last
+    std::vector<int> ia, ja;
+    std::vector<double> a;
+    std::vector<int> starts;    // P+1
+    std::vector<int> ridx;      // n: reduced index, -1 if not a dependent
+    std::vector<int> dep_rows;  // dependents in ascending order (global rows)
+    std::vector<Part> parts;
+    std::vector<int> kcols;     // concatenated K_p, global columns ascending
+    // coupling rows for the pass-2 right-hand-side update b' = b - C x_K
+    std::vector<int> cpl_rows;  // global rows, grouped by partition, ascending
+    std::vector<int> cpl_ptr;   // CSR over cpl_rows
+    std::vector<int> cpl_col;   // reduced index of the coupling column
+    std::vector<double> cpl_val;
+    StepStream pass1, pass2;
+    int64_t spike_area = 0;
+};
+
+struct PlanOptions {
+    int P = 1;
+    int lanes_per_row = 0;      // 0 = automatic
+    int kmax_limit = 16384;     // largest partition a workgroup can hold in LDS
+};
+
+// Builds the plan.  Returns an empty string on success, else an error message.
+// The input CSR is validated (ia[0] == 0, no empty rows, strictly ascending
+// columns, diagonal last for 'L' / first for 'U', n >= 1, 1 <= P <= n).
+std::string build_plan(char uplo, int n, const double *a, const int *ia, const int *ja,
+                       const PlanOptions &opt, Plan &plan);
+
+// Chooses lanes-per-row G for a pass from the row-length statistics.
+int choose_lanes(const std::vector<int> &row_len);
+
+// Level (longest dependency chain, 0-based) of each row of a lower triangular
+// CSR given as (rp, ci) over rows [0, m); entries must point to earlier rows.
+std::vector<int> row_levels(int m, const int *rp, const int *ci);
+
+}  // namespace spike

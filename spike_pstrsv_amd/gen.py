@@ -1,0 +1,115 @@
+"""Deterministic synthetic workloads (binding of libspike_gen.so, csrc/gen.c).
+
+All matrices are lower-triangular CSR (diagonal last) with int32 indices and
+float64 values; `to_upper` gives the index-reversed upper-triangular form
+(diagonal first) of the same system.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libspike_gen.so"
+if not _LIB_PATH.exists():
+    raise ImportError(f"{_LIB_PATH} is not built; run spike_pstrsv_amd/build.py")
+_g = C.CDLL(str(_LIB_PATH))
+_vp, _i64, _u64, _d, _i = C.c_void_p, C.c_int64, C.c_uint64, C.c_double, C.c_int
+_g.spike_gen_lap2d_nnz.argtypes = [_i]; _g.spike_gen_lap2d_nnz.restype = _i64
+_g.spike_gen_lap2d.argtypes = [_i, _vp, _vp, _vp]; _g.spike_gen_lap2d.restype = _i64
+_g.spike_gen_lap3d_nnz.argtypes = [_i]; _g.spike_gen_lap3d_nnz.restype = _i64
+_g.spike_gen_lap3d.argtypes = [_i, _vp, _vp, _vp]; _g.spike_gen_lap3d.restype = _i64
+_g.spike_gen_band_count.argtypes = [_i64, _i, _i, _d, _u64, _vp]; _g.spike_gen_band_count.restype = _i64
+_g.spike_gen_band_fill.argtypes = [_i64, _i, _i, _d, _u64, _vp, _vp, _vp, _vp]; _g.spike_gen_band_fill.restype = _i64
+_g.spike_gen_rhs.argtypes = [_i64, _i, _i64, _u64, _vp]; _g.spike_gen_rhs.restype = None
+_g.spike_gen_params.argtypes = [_u64, _u64, _i64, _vp]; _g.spike_gen_params.restype = None
+
+# seed family: 0x5350494B45000000 + workload number
+SEED_BASE = 0x5350494B45000000
+
+
+@dataclass
+class CSR:
+    a: np.ndarray
+    ia: np.ndarray
+    ja: np.ndarray
+    uplo: str = "L"
+
+    @property
+    def n(self) -> int:
+        return len(self.ia) - 1
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ia[-1])
+
+
+def lap2d(g: int) -> CSR:
+    n = g * g
+    nnz = _g.spike_gen_lap2d_nnz(g)
+    ia = np.empty(n + 1, np.int32); ja = np.empty(nnz, np.int32); a = np.empty(nnz, np.float64)
+    _g.spike_gen_lap2d(g, ia.ctypes.data, ja.ctypes.data, a.ctypes.data)
+    return CSR(a, ia, ja)
+
+
+def lap3d(g: int) -> CSR:
+    n = g * g * g
+    nnz = _g.spike_gen_lap3d_nnz(g)
+    ia = np.empty(n + 1, np.int32); ja = np.empty(nnz, np.int32); a = np.empty(nnz, np.float64)
+    _g.spike_gen_lap3d(g, ia.ctypes.data, ja.ctypes.data, a.ctypes.data)
+    return CSR(a, ia, ja)
+
+
+def banded(n: int, bw: int = 64, q: int = 16, plr: float = 0.02, seed: int = SEED_BASE + 2) -> CSR:
+    rowlen = np.empty(n, np.int32)
+    nnz = _g.spike_gen_band_count(n, bw, q, plr, seed, rowlen.ctypes.data)
+    if nnz < 0:
+        raise ValueError("invalid banded parameters")
+    ia = np.empty(n + 1, np.int32); ja = np.empty(nnz, np.int32); a = np.empty(nnz, np.float64)
+    _g.spike_gen_band_fill(n, bw, q, plr, seed, rowlen.ctypes.data, ia.ctypes.data, ja.ctypes.data, a.ctypes.data)
+    return CSR(a, ia, ja)
+
+
+def rhs(n: int, nrhs: int = 1, seed: int = SEED_BASE + 100) -> np.ndarray:
+    """Column-major right-hand sides: shape (nrhs, n) in memory order, entries U(-1,1)."""
+    b = np.empty((nrhs, n), np.float64)
+    _g.spike_gen_rhs(n, nrhs, n, seed, b.ctypes.data)
+    return b if nrhs > 1 else b[0]
+
+
+def params(seed: int, stream: int, count: int) -> np.ndarray:
+    out = np.empty(count, np.float64)
+    _g.spike_gen_params(seed, stream, count, out.ctypes.data)
+    return out
+
+
+def to_upper(m: CSR) -> CSR:
+    """Index reversal U = J L J: the upper-triangular system with U (J x) = J b."""
+    n = m.n
+    lens = np.diff(m.ia)[::-1]
+    ia = np.zeros(n + 1, np.int32)
+    np.cumsum(lens, out=ia[1:])
+    # rows reversed, entries within each row reversed == the whole entry array reversed
+    ja = (n - 1 - m.ja[::-1]).astype(np.int32)
+    a = m.a[::-1].copy()
+    return CSR(a, ia, ja, "U")
+
+
+def spmv(m: CSR, x: np.ndarray) -> np.ndarray:
+    rows = np.repeat(np.arange(m.n), np.diff(m.ia))
+    y = np.zeros(m.n)
+    np.add.at(y, rows, m.a * x[m.ja])
+    return y
+
+
+def workload(name: str) -> tuple[CSR, int]:
+    """Named workloads with their partition counts."""
+    if name == "lap2d_256":
+        return lap2d(256), 8
+    if name == "banded_1m":
+        return banded(1 << 20), 1024
+    if name == "lap3d_128":
+        return lap3d(128), 0
+    raise KeyError(name)

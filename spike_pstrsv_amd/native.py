@@ -1,0 +1,213 @@
+"""ctypes binding of libspike_pstrsv.so (the C ABI in include/spike_pstrsv.h).
+
+Mirrors the reference's C interface (src/spike_pstrsv.h:7-38: spike_pstrsv,
+spike_pstrsv_preproc, spike_pstrsv_solve and the profiling twins) plus the
+extension API (factor handles, multi-RHS, device pointers, stats).  Arrays are
+numpy arrays (host) or integer device addresses (e.g. torch.Tensor.data_ptr()).
+There is no fallback: if the native library is missing, importing this module
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libspike_pstrsv.so"
+
+SPIKE_OK = 0
+ERRORS = {1: "SPIKE_ERR_ARG", 2: "SPIKE_ERR_MATRIX", 3: "SPIKE_ERR_DEVICE", 4: "SPIKE_ERR_NOMEM",
+          5: "SPIKE_ERR_STATE", 6: "SPIKE_ERR_UNSUPPORTED"}
+
+# Every symbol declared in include/spike_pstrsv.h.
+EXPORTED = [
+    "spike_pstrsv_preproc", "_spike_pstrsv_preproc", "spike_pstrsv", "spike_pstrsv_solve", "_spike_pstrsv_solve",
+    "spike_options_default", "spike_factor_create", "spike_factor_solve", "spike_factor_destroy",
+    "spike_factor_stats", "spike_factor_log", "spike_analyze", "spike_pstrsv_set_partitions",
+    "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
+]
+
+
+class Options(C.Structure):
+    _fields_ = [("partitions", C.c_int), ("drop_tol", C.c_double), ("precision", C.c_int),
+                ("lanes_per_row", C.c_int), ("verbose", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n", C.c_int), ("partitions", C.c_int), ("nnz", C.c_int64), ("n_reduced", C.c_int),
+                ("nnz_reduced", C.c_int64), ("reduced_levels", C.c_int), ("spike_area", C.c_int64),
+                ("dropped_rowsum_max", C.c_double), ("steps_pass1", C.c_int), ("steps_pass2", C.c_int),
+                ("lanes_per_row", C.c_int), ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SpikeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+if not LIB_PATH.exists():
+    raise ImportError(f"{LIB_PATH} is not built; run spike_pstrsv_amd/build.py (or __graft_entry__.build())")
+
+lib = C.CDLL(str(LIB_PATH))
+_vp, _i, _d, _c = C.c_void_p, C.c_int, C.c_double, C.c_char
+lib.spike_pstrsv_preproc.argtypes = [_vp, _vp, _vp, _i, _i, _c]
+lib._spike_pstrsv_preproc.argtypes = [_vp, _vp, _vp, _i, _i, _c]
+lib.spike_pstrsv.argtypes = [_c, _i, _vp, _vp, _vp, _vp, _vp, _i]
+lib.spike_pstrsv_solve.argtypes = [_c, _i, _vp, _vp, _i]
+lib._spike_pstrsv_solve.argtypes = [_c, _i, _vp, _vp, _i]
+for _f in ("spike_pstrsv_preproc", "_spike_pstrsv_preproc", "spike_pstrsv", "spike_pstrsv_solve",
+           "_spike_pstrsv_solve"):
+    getattr(lib, _f).restype = None
+lib.spike_options_default.argtypes = [C.POINTER(Options)]
+lib.spike_options_default.restype = None
+lib.spike_factor_create.argtypes = [C.POINTER(_vp), _c, _i, _vp, _vp, _vp, C.POINTER(Options)]
+lib.spike_factor_create.restype = _i
+lib.spike_factor_solve.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
+lib.spike_factor_solve.restype = _i
+lib.spike_factor_destroy.argtypes = [_vp]
+lib.spike_factor_destroy.restype = _i
+lib.spike_factor_stats.argtypes = [_vp, C.POINTER(Stats)]
+lib.spike_factor_stats.restype = _i
+lib.spike_factor_log.argtypes = [_vp]
+lib.spike_factor_log.restype = _i
+lib.spike_analyze.argtypes = [_c, _i, _vp, _vp, _vp, C.POINTER(Options), C.POINTER(Stats)]
+lib.spike_analyze.restype = _i
+lib.spike_pstrsv_set_partitions.argtypes = [_i]
+lib.spike_pstrsv_set_partitions.restype = None
+lib.spike_pstrsv_current_factor.argtypes = []
+lib.spike_pstrsv_current_factor.restype = _vp
+lib.spike_last_error.argtypes = []
+lib.spike_last_error.restype = C.c_char_p
+lib.spike_pstrsv_get_timers.argtypes = [C.POINTER(_d), _i]
+lib.spike_pstrsv_get_timers.restype = _i
+
+
+def last_error() -> str:
+    return lib.spike_last_error().decode()
+
+
+def _check(rc: int) -> None:
+    if rc != SPIKE_OK:
+        raise SpikeError(rc, last_error())
+
+
+def _ptr(a) -> int:
+    """Address of a numpy array (host) or an int device address."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    if isinstance(a, int):
+        return a
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    raise TypeError(f"cannot take the address of {type(a)}")
+
+
+def _csr(a, ia, ja):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    ia = np.ascontiguousarray(ia, dtype=np.int32)
+    ja = np.ascontiguousarray(ja, dtype=np.int32)
+    return a, ia, ja
+
+
+def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False) -> Options:
+    o = Options()
+    lib.spike_options_default(C.byref(o))
+    o.partitions = partitions
+    o.drop_tol = drop_tol
+    o.lanes_per_row = lanes_per_row
+    o.verbose = int(verbose)
+    return o
+
+
+class Factor:
+    """A preprocessed matrix resident in device memory (spike_factor_t)."""
+
+    def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
+                 lanes_per_row: int = 0, verbose: bool = False):
+        a, ia, ja = _csr(a, ia, ja)
+        self.n = len(ia) - 1
+        h = _vp()
+        o = options(partitions, drop_tol, lanes_per_row, verbose)
+        _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
+                                       ja.ctypes.data, C.byref(o)))
+        self.handle = h.value
+
+    def solve(self, b, x=None, nrhs: int = 1, ldb: int | None = None, ldx: int | None = None, stream: int = 0):
+        """Solves A X = B.  b/x: numpy arrays (column-major for nrhs > 1) or device addresses."""
+        n = self.n
+        ldb = n if ldb is None else ldb
+        ldx = n if ldx is None else ldx
+        if x is None:
+            x = np.zeros((nrhs, ldx) if nrhs > 1 else ldx, dtype=np.float64)
+        _check(lib.spike_factor_solve(self.handle, nrhs, _ptr(b), ldb, _ptr(x), ldx, stream or None))
+        return x
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(lib.spike_factor_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def log(self) -> None:
+        _check(lib.spike_factor_log(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.spike_factor_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def analyze(uplo: str, a, ia, ja, partitions: int = 0, lanes_per_row: int = 0) -> dict:
+    """Host-only symbolic analysis (no device needed)."""
+    a, ia, ja = _csr(a, ia, ja)
+    st = Stats()
+    o = options(partitions, -1.0, lanes_per_row)
+    _check(lib.spike_analyze(uplo.encode()[:1], len(ia) - 1, a.ctypes.data, ia.ctypes.data, ja.ctypes.data,
+                             C.byref(o), C.byref(st)))
+    return st.as_dict()
+
+
+# ---- legacy entry points (reference src/spike_pstrsv.h) -------------------
+
+def spike_pstrsv(uplo: str, m: int, a, ia, ja, b, x, nthreads: int) -> None:
+    lib.spike_pstrsv(uplo.encode()[:1], m, _ptr(a), _ptr(ia), _ptr(ja), _ptr(b), _ptr(x), nthreads)
+
+
+def spike_pstrsv_preproc(a, ia, ja, n: int, nthreads: int, uplo: str, profile: bool = False) -> None:
+    f = lib._spike_pstrsv_preproc if profile else lib.spike_pstrsv_preproc
+    f(_ptr(a), _ptr(ia), _ptr(ja), n, nthreads, uplo.encode()[:1])
+
+
+def spike_pstrsv_solve(uplo: str, m: int, b, x, nthreads: int, profile: bool = False) -> None:
+    f = lib._spike_pstrsv_solve if profile else lib.spike_pstrsv_solve
+    f(uplo.encode()[:1], m, _ptr(b), _ptr(x), nthreads)
+
+
+def set_partitions(P: int) -> None:
+    lib.spike_pstrsv_set_partitions(P)
+
+
+def timers() -> list[float]:
+    buf = (C.c_double * 9)()
+    k = lib.spike_pstrsv_get_timers(buf, 9)
+    return list(buf[:k])
+
+
+def current_stats() -> dict | None:
+    h = lib.spike_pstrsv_current_factor()
+    if not h:
+        return None
+    st = Stats()
+    _check(lib.spike_factor_stats(h, C.byref(st)))
+    return st.as_dict()
