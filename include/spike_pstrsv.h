@@ -93,7 +93,18 @@ typedef struct {
     int lanes_per_row;        /* block-solve lanes per row (1..64, power of 2);*/
                               /* 0 = automatic                                 */
     int verbose;              /* print the partition log after preprocessing   */
+    int reduced_solver;       /* SPIKE_REDUCED_*                               */
 } spike_options_t;
+
+/* Reduced-system solvers. */
+enum {
+    SPIKE_REDUCED_AUTO = 0,   /* cheaper of the two by a cost model            */
+    SPIKE_REDUCED_EXACT = 1,  /* level-scheduled forward substitution, one     */
+                              /* workgroup                                     */
+    SPIKE_REDUCED_JACOBI = 2  /* S Jacobi sweeps x = y - W x over the whole    */
+                              /* GPU; S is fixed at preprocessing so that the  */
+                              /* bound sum_{j>S} max(|W|^j 1) <= 1e-16 holds   */
+};
 
 /* Fills opts with the defaults. */
 void spike_options_default(spike_options_t *opts);
@@ -109,6 +120,12 @@ int spike_factor_create(spike_factor_t *f, char uplo, int n, const double *a,
  * asynchronous with respect to the host, for host pointers it synchronises. */
 int spike_factor_solve(spike_factor_t f, int nrhs, const double *b, int ldb,
                        double *x, int ldx, void *stream);
+
+/* Same as spike_factor_solve, and adds the device time of each phase, in
+ * milliseconds, to phase_ms[0..3]: pass-1 block solve, reduced solve, coupling
+ * update, pass-2 block solve (hipEvents recorded on stream).  Synchronises. */
+int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int ldb,
+                                double *x, int ldx, void *stream, double *phase_ms);
 
 int spike_factor_destroy(spike_factor_t f);
 
@@ -126,6 +143,8 @@ typedef struct {
     int lanes_per_row;        /* block-solve lanes per row in use              */
     int64_t device_bytes;     /* device memory held by the factor              */
     double preproc_seconds;   /* wall time of spike_factor_create              */
+    int reduced_sweeps;       /* Jacobi sweeps per solve, 0 = exact solver     */
+    double jacobi_bound;      /* bound on the reduced solve's relative error   */
 } spike_stats_t;
 
 int spike_factor_stats(spike_factor_t f, spike_stats_t *st);

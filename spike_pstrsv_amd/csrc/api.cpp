@@ -62,12 +62,13 @@ struct spike_factor_s {
     int *d_dep_rows = nullptr;
     int *d_cpl_rows = nullptr, *d_cpl_ptr = nullptr, *d_cpl_col = nullptr;
     double *d_cpl_val = nullptr;
-    spike::DevSteps s1{}, s2{};
+    spike::DevSteps steps{};
     int *d_rrp = nullptr, *d_rci = nullptr, *d_rlist = nullptr, *d_levptr = nullptr;
     double *d_rv = nullptr;
     int nlev = 0;
     bool red_in_lds = false;
-    double *d_xred = nullptr, *d_bc = nullptr;
+    int sweeps = 0;             // > 0: Jacobi sweeps, 0: exact level solve
+    double *d_y = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_bc = nullptr;
     // staging for host right-hand sides / solutions
     double *d_stage_b = nullptr, *d_stage_x = nullptr;
     size_t stage_cap = 0;
@@ -166,10 +167,12 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         (rc = f->upload(&f->d_cpl_rows, pl.cpl_rows)) || (rc = f->upload(&f->d_cpl_ptr, pl.cpl_ptr)) ||
         (rc = f->upload(&f->d_cpl_col, pl.cpl_col)) || (rc = f->upload(&f->d_cpl_val, pl.cpl_val)))
         return rc;
-    if ((rc = upload_steps(f, pl.pass1, f->s1)) || (rc = upload_steps(f, pl.pass2, f->s2))) return rc;
+    if ((rc = upload_steps(f, pl.steps, f->steps))) return rc;
 
     const int nred = (int)pl.dep_rows.size();
-    if ((rc = f->dalloc(&f->d_xred, nred)) || (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size()))) return rc;
+    if ((rc = f->dalloc(&f->d_y, nred)) || (rc = f->dalloc(&f->d_xa, nred)) || (rc = f->dalloc(&f->d_xb, nred)) ||
+        (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size())))
+        return rc;
 
     // ---- spikes on the device
     std::vector<spike::SpikePart> sp;
@@ -185,6 +188,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     }
     std::vector<int> hrp(nred + 1, 0);
     std::vector<int> hci;
+    std::vector<double> hv;
     double dropmax = 0.0;
     int64_t rnnz = 0;
     double t2 = now_s(), t3 = t2;
@@ -243,7 +247,11 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         HIP_TRY(spike::launch_reduced_fill(nred, d_wrow, d_qkb, d_qk, d_tol, d_W, d_kridx, f->d_rrp, f->d_rci,
                                            f->d_rv, nullptr));
         hci.resize(rnnz);
-        if (rnnz) HIP_TRY(hipMemcpy(hci.data(), f->d_rci, sizeof(int) * rnnz, hipMemcpyDeviceToHost));
+        hv.resize(rnnz);
+        if (rnnz) {
+            HIP_TRY(hipMemcpy(hci.data(), f->d_rci, sizeof(int) * rnnz, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hv.data(), f->d_rv, sizeof(double) * rnnz, hipMemcpyDeviceToHost));
+        }
         HIP_TRY(hipDeviceSynchronize());
         dfree(f, d_W, sizeof(double) * std::max<int64_t>(area, 1));
         dfree(f, d_a, sizeof(double) * pl.a.size());
@@ -274,11 +282,66 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     if ((rc = f->upload(&f->d_levptr, levptr)) || (rc = f->upload(&f->d_rlist, rlist))) return rc;
     f->red_in_lds = (size_t)nred * sizeof(double) <= spike::block_solve_lds_limit();
 
+    // Jacobi sweep count: after S sweeps from x = y the error is
+    // sum_{j>S} (-W)^j y, bounded entry-wise by ||y||_inf sum_{j>S} mx_j with
+    // mx_j = max_i (|W|^j 1)_i.  mx is submultiplicative (mx_{a+b} <=
+    // mx_a mx_b for the nonnegative |W|), so with J computed terms and
+    // mx_J < 1 the terms beyond J sum to at most
+    // mx_J / (1 - mx_J) * (1 + mx_1 + ... + mx_{J-1}).  W is nilpotent, so
+    // terms past its level count vanish exactly.
+    int sweeps = -1;
+    double jbound = 0.0;
+    if (rnnz > 0) {
+        std::vector<double> r(nred, 1.0), r2(nred);
+        std::vector<double> mx;   // mx[j-1] = mx_j
+        bool exact_end = false;
+        for (int j = 1; j <= 64; j++) {
+            double m = 0.0;
+            for (int q = 0; q < nred; q++) {
+                double acc = 0.0;
+                for (int e = hrp[q]; e < hrp[q + 1]; e++) acc += std::fabs(hv[e]) * r[hci[e]];
+                r2[q] = acc;
+                m = std::max(m, acc);
+            }
+            mx.push_back(m);
+            r.swap(r2);
+            if (m == 0.0 || j >= maxlev) { exact_end = true; break; }
+            if (m < 1e-20) break;
+        }
+        const int J = (int)mx.size();
+        double beyond = 0.0;
+        if (!exact_end) {
+            double partial = 1.0;
+            for (int j = 0; j < J - 1; j++) partial += mx[j];
+            beyond = mx[J - 1] < 1.0 ? mx[J - 1] / (1.0 - mx[J - 1]) * partial : 1e300;
+        }
+        // smallest S with sum_{j>S} mx_j <= 1e-16
+        double tail = beyond;
+        if (tail <= 1e-16) {
+            sweeps = J;
+            jbound = tail;
+            for (int S = J - 1; S >= 0; S--) {
+                tail += mx[S];           // mx[S] is the term j = S+1
+                if (tail > 1e-16) break;
+                sweeps = S;
+                jbound = tail;
+            }
+        }
+    }
+    // cost model (microseconds): level solve ~0.15 per level in LDS, ~1 in
+    // L2; a sweep ~2.5 launch + streaming the reduced system at 4 TB/s
+    const double t_exact = maxlev * (f->red_in_lds ? 0.15 : 1.0);
+    const double t_jacobi = sweeps < 0 ? 1e30 : sweeps * (2.5 + (double)rnnz * 12.0 / 4e6);
+    int mode = f->opts.reduced_solver;
+    if (mode == SPIKE_REDUCED_JACOBI && sweeps < 0)
+        return fail(SPIKE_ERR_UNSUPPORTED, "Jacobi sweeps do not reach the reduced-system tolerance in 64 terms");
+    if (mode == SPIKE_REDUCED_AUTO) mode = (rnnz > 0 && t_jacobi < t_exact) ? SPIKE_REDUCED_JACOBI : SPIKE_REDUCED_EXACT;
+    f->sweeps = (mode == SPIKE_REDUCED_JACOBI && rnnz > 0) ? std::max(sweeps, 1) : 0;
+
     // host copies no longer needed
     pl.a.clear(); pl.a.shrink_to_fit();
     pl.ja.clear(); pl.ja.shrink_to_fit();
-    pl.pass1.val.clear(); pl.pass1.val.shrink_to_fit(); pl.pass1.col.clear(); pl.pass1.col.shrink_to_fit();
-    pl.pass2.val.clear(); pl.pass2.val.shrink_to_fit(); pl.pass2.col.clear(); pl.pass2.col.shrink_to_fit();
+    pl.steps.val.clear(); pl.steps.val.shrink_to_fit(); pl.steps.col.clear(); pl.steps.col.shrink_to_fit();
     pl.cpl_val.clear(); pl.cpl_val.shrink_to_fit();
 
     spike_stats_t &st = f->stats;
@@ -290,9 +353,11 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.reduced_levels = maxlev;
     st.spike_area = pl.spike_area;
     st.dropped_rowsum_max = dropmax;
-    st.steps_pass1 = pl.pass1.max_steps;
-    st.steps_pass2 = pl.pass2.max_steps;
-    st.lanes_per_row = pl.pass1.G;
+    st.steps_pass1 = pl.steps.max_steps;
+    st.steps_pass2 = pl.steps.max_steps;
+    st.lanes_per_row = pl.steps.G;
+    st.reduced_sweeps = f->sweeps;
+    st.jacobi_bound = f->sweeps > 0 ? jbound : 0.0;
     st.device_bytes = f->device_bytes;
     st.preproc_seconds = now_s() - t0;
     g_timers[0] += st.preproc_seconds;
@@ -318,9 +383,10 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
     const Plan &pl = f->plan;
     spike::BlockArgs ba;
     ba.parts = f->d_parts;
+    ba.st = f->steps;
     ba.b = b;
     ba.x = x;
-    ba.xred = f->d_xred;
+    ba.xred = f->d_y;
     ba.bc = f->d_bc;
     ba.dep_rows = f->d_dep_rows;
     ba.cpl_rows = f->d_cpl_rows;
@@ -328,18 +394,27 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
     ba.flip = pl.flip ? 1 : 0;
     ba.kmax = pl.kmax;
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
-    ba.st = f->s1;
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));
     const int nred = (int)pl.dep_rows.size();
     if (nred > 0) {
-        HIP_TRY(spike::launch_reduced_solve(nred, f->nlev, f->d_levptr, f->d_rlist, f->d_rrp, f->d_rci, f->d_rv,
-                                            f->d_xred, f->red_in_lds, s));
+        const double *xr = f->d_y;
+        if (f->sweeps > 0) {
+            const double *xin = f->d_y;
+            for (int m = 1; m <= f->sweeps; m++) {
+                double *xout = (m & 1) ? f->d_xa : f->d_xb;
+                HIP_TRY(spike::launch_jacobi_sweep(nred, f->d_rrp, f->d_rci, f->d_rv, f->d_y, xin, xout, s));
+                xin = xout;
+            }
+            xr = xin;
+        } else {
+            HIP_TRY(spike::launch_reduced_solve(nred, f->nlev, f->d_levptr, f->d_rlist, f->d_rrp, f->d_rci,
+                                                f->d_rv, f->d_y, f->red_in_lds, s));
+        }
         if (ev) HIP_TRY(hipEventRecord(ev[2], s));
         HIP_TRY(spike::launch_coupling((int)pl.cpl_rows.size(), f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
-                                       f->d_cpl_val, f->d_xred, b, f->d_bc, pl.n, ba.flip, s));
+                                       f->d_cpl_val, xr, b, f->d_bc, pl.n, ba.flip, s));
         if (ev) HIP_TRY(hipEventRecord(ev[3], s));
-        ba.st = f->s2;
         HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
     } else if (ev) {
         HIP_TRY(hipEventRecord(ev[2], s));
@@ -350,7 +425,7 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
 }
 
 int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, int ldx, hipStream_t s,
-                 bool timed)
+                 bool timed, double *phase_ms = nullptr)
 {
     const int n = f->plan.n;
     if (nrhs < 0 || ldb < n || ldx < n) return fail(SPIKE_ERR_ARG, "nrhs must be >= 0 and ldb, ldx >= n");
@@ -395,6 +470,8 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
             HIP_TRY(hipEventSynchronize(ev[4]));
             float ms[4];
             for (int i = 0; i < 4; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+            if (phase_ms)
+                for (int i = 0; i < 4; i++) phase_ms[i] += ms[i];
             g_timers[5] += ms[0] * 1e-3;
             g_timers[6] += ms[1] * 1e-3;
             g_timers[7] += ms[2] * 1e-3;
@@ -483,6 +560,7 @@ void spike_options_default(spike_options_t *o)
     o->precision = SPIKE_PREC_FP64;
     o->lanes_per_row = 0;
     o->verbose = 0;
+    o->reduced_solver = SPIKE_REDUCED_AUTO;
 }
 
 int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, const int *ia, const int *ja,
@@ -516,6 +594,14 @@ int spike_factor_solve(spike_factor_t f, int nrhs, const double *b, int ldb, dou
 {
     if (!f) return fail(SPIKE_ERR_STATE, "factor is NULL");
     return factor_solve(f, nrhs, b, ldb, x, ldx, (hipStream_t)stream, false);
+}
+
+int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int ldb, double *x, int ldx,
+                                void *stream, double *phase_ms)
+{
+    if (!f) return fail(SPIKE_ERR_STATE, "factor is NULL");
+    if (!phase_ms) return fail(SPIKE_ERR_ARG, "phase_ms is NULL");
+    return factor_solve(f, nrhs, b, ldb, x, ldx, (hipStream_t)stream, true, phase_ms);
 }
 
 int spike_factor_destroy(spike_factor_t f)
@@ -580,9 +666,9 @@ int spike_analyze(char uplo, int n, const double *a, const int *ia, const int *j
     st->nnz = pl.nnz;
     st->n_reduced = (int)pl.dep_rows.size();
     st->spike_area = pl.spike_area;
-    st->steps_pass1 = pl.pass1.max_steps;
-    st->steps_pass2 = pl.pass2.max_steps;
-    st->lanes_per_row = pl.pass1.G;
+    st->steps_pass1 = pl.steps.max_steps;
+    st->steps_pass2 = pl.steps.max_steps;
+    st->lanes_per_row = pl.steps.G;
     return SPIKE_OK;
 }
 

@@ -53,29 +53,16 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const int lane = threadIdx.x;
     const int slot = lane / G, li = lane % G;
     const int s = pt.s, k = pt.k;
-    int lo, hi;
-    if (PASS == 1) {
-        if (pt.hi1 < 0) return;
-        lo = 0;
-        hi = pt.hi1;
-    } else {
-        if (pt.f >= k) return;
-        lo = pt.f;
-        hi = k - 1;
-    }
+    // Pass 1 is needed when the partition holds dependents or has no coupling
+    // (then its pass-1 result is final); pass 2 only when it has coupling.
+    if (PASS == 1 && pt.hi1 < 0) return;
+    if (PASS == 2 && pt.f >= k) return;
     const int n = A.n, flip = A.flip;
 
-    // right-hand side (pass 2: rows before f are final from pass 1)
-    if (PASS == 1) {
-        for (int r = lane; r <= hi; r += 64) {
-            int g = s + r;
-            xs[r] = A.b[flip ? n - 1 - g : g];
-        }
-    } else {
-        for (int r = lane; r < k; r += 64) {
-            int g = s + r, m = flip ? n - 1 - g : g;
-            xs[r] = r < lo ? A.x[m] : A.b[m];
-        }
+    // right-hand side; pass 2 replaces it by b - C x_K on the coupling rows
+    for (int r = lane; r < k; r += 64) {
+        const int g = s + r;
+        xs[r] = A.b[flip ? n - 1 - g : g];
     }
     if (lane == 0) xs[k] = 0.0;
     if (PASS == 2) {
@@ -162,19 +149,14 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 epilogue:
     __syncthreads();
 
-    if (PASS == 1) {
+    if (PASS == 1)
         for (int q = lane; q < pt.dcnt; q += 64) A.xred[pt.dbeg + q] = xs[A.dep_rows[pt.dbeg + q] - s];
-        const int fin = pt.f < hi + 1 ? pt.f : hi + 1;   // rows before f are final
-        for (int r = lane; r < fin; r += 64) {
-            int g = s + r;
+    // pass 2 writes the solution; without coupling, pass 1's result is final
+    if (PASS == 2 || pt.f >= k)
+        for (int r = lane; r < k; r += 64) {
+            const int g = s + r;
             A.x[flip ? n - 1 - g : g] = xs[r];
         }
-    } else {
-        for (int r = lo + lane; r < k; r += 64) {
-            int g = s + r;
-            A.x[flip ? n - 1 - g : g] = xs[r];
-        }
-    }
 }
 
 template <int G, int PASS>
@@ -232,6 +214,25 @@ __global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, cons
     }
     if (IN_LDS)
         for (int i = tid; i < nred; i += nt) xred[i] = xl[i];
+}
+
+// One Jacobi sweep of the unit lower reduced system x = y - W x:
+// xout = y - W xin, G lanes per row, fixed summation order.
+template <int G>
+__global__ __launch_bounds__(256) void k_jacobi_sweep(int nred, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci, const double *__restrict__ v,
+                                                      const double *__restrict__ y, const double *__restrict__ xin,
+                                                      double *__restrict__ xout)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = tid / G, li = tid % G;
+    double acc = 0.0;
+    if (row < nred) {
+        const int e = rp[row + 1];
+        for (int j = rp[row] + li; j < e; j += G) acc = fma(v[j], xin[ci[j]], acc);
+    }
+    acc = group_sum<G>(acc);
+    if (li == 0 && row < nred) xout[row] = y[row] - acc;
 }
 
 // ---------------------------------------------------------------- coupling
@@ -382,6 +383,17 @@ hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int
         hipLaunchKernelGGL(k_reduced_solve<false>, dim3(1), dim3(1024), 0, s, nred, nlev, levptr, rlist, rp, ci, v,
                            xred);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_sweep(int nred, const int *rp, const int *ci, const double *v, const double *y,
+                               const double *xin, double *xout, hipStream_t s)
+{
+    if (nred <= 0) return hipSuccess;
+    constexpr int G = 8;
+    const long threads = (long)nred * G;
+    hipLaunchKernelGGL(k_jacobi_sweep<G>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, nred, rp, ci, v,
+                       y, xin, xout);
     return hipGetLastError();
 }
 

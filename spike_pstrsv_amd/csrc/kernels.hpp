@@ -10,7 +10,7 @@ namespace spike {
 struct DevPart {
     int s, k;       // rows [s, s+k) of the lower-triangular form
     int f;          // first coupling row (local), k if none
-    int hi1;        // pass 1 solves local rows [0, hi1]; -1 = nothing
+    int hi1;        // < 0: the partition needs no pass 1
     int dbeg, dcnt; // dependents (reduced indices dbeg .. dbeg+dcnt)
     int cbeg, ccnt; // coupling rows (indices into the coupling arrays)
 };
@@ -39,8 +39,9 @@ struct BlockArgs {
     int kmax;               // largest partition (LDS x segment = kmax+1 doubles)
 };
 
-// Pass 1 (y = D^-1 b on rows [0, hi1]) or pass 2 (re-solve rows [f, k) with
-// the coupling-corrected right-hand side), one wavefront per partition.
+// Pass 1 (y = D^-1 b, written at the dependents; also the solution of
+// partitions without coupling) or pass 2 (re-solve with the coupling-corrected
+// right-hand side), one wavefront per partition.
 hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s);
 
 // Single-workgroup forward solve of the unit lower reduced system, in place on
@@ -49,6 +50,10 @@ hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s
 hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int *rlist,
                                 const int *rp, const int *ci, const double *v,
                                 double *xred, bool in_lds, hipStream_t s);
+
+// One Jacobi sweep xout = y - W xin of the unit lower reduced system.
+hipError_t launch_jacobi_sweep(int nred, const int *rp, const int *ci, const double *v, const double *y,
+                               const double *xin, double *xout, hipStream_t s);
 
 // bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c.
 hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,

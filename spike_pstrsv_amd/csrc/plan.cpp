@@ -324,48 +324,41 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     int G = opt.lanes_per_row > 0 ? opt.lanes_per_row : choose_lanes(inblock_len);
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) return "lanes_per_row must be 1, 2, 4, 8 or 16";
     const int M = 1;
-    for (int pass = 1; pass <= 2; pass++) {
-        StepStream &ss = pass == 1 ? pl.pass1 : pl.pass2;
-        ss.G = G; ss.M = M; ss.R = 64 / G;
-        std::vector<std::vector<double>> pv(P), pm(P);
-        std::vector<std::vector<uint16_t>> pc(P), pr(P);
-        std::vector<int> pn(P, 0);
-        parallel_for(P, [&](int t) {
-            const Part &pt = pl.parts[t];
-            int lo, hi;
-            if (pass == 1) { lo = 0; hi = pt.hi1; }
-            else { lo = pt.f; hi = pt.k - 1; }
-            if (hi < lo) return;
-            std::vector<int> rb(pt.k + 1, 0), ecol;
-            std::vector<double> ev, dg(pt.k);
-            for (int r = 0; r < pt.k; r++) {
-                int i = pt.s + r;
-                if (r >= lo && r <= hi)
-                    for (int j = LA[i]; j < LA[i + 1] - 1; j++)
-                        if (LJ[j] >= pt.s) { ecol.push_back(LJ[j] - pt.s); ev.push_back(LV[j]); }
-                rb[r + 1] = (int)ecol.size();
-                dg[r] = LV[LA[i + 1] - 1];
-            }
-            schedule_partition(pt.k, lo, hi, rb, ecol, ev, dg, G, M, pv[t], pc[t], pr[t], pm[t], pn[t]);
-        });
-        ss.part_begin.assign(P + 1, 0);
-        for (int t = 0; t < P; t++) {
-            ss.part_begin[t + 1] = ss.part_begin[t] + pn[t];
-            ss.max_steps = std::max(ss.max_steps, pn[t]);
+    StepStream &ss = pl.steps;
+    ss.G = G; ss.M = M; ss.R = 64 / G;
+    std::vector<std::vector<double>> pv(P), pm(P);
+    std::vector<std::vector<uint16_t>> pc(P), pr(P);
+    std::vector<int> pn(P, 0);
+    parallel_for(P, [&](int t) {
+        const Part &pt = pl.parts[t];
+        std::vector<int> rb(pt.k + 1, 0), ecol;
+        std::vector<double> ev, dg(pt.k);
+        for (int r = 0; r < pt.k; r++) {
+            int i = pt.s + r;
+            for (int j = LA[i]; j < LA[i + 1] - 1; j++)
+                if (LJ[j] >= pt.s) { ecol.push_back(LJ[j] - pt.s); ev.push_back(LV[j]); }
+            rb[r + 1] = (int)ecol.size();
+            dg[r] = LV[LA[i + 1] - 1];
         }
-        ss.nsteps = ss.part_begin[P];
-        ss.val.resize((size_t)ss.nsteps * M * 64);
-        ss.col.resize((size_t)ss.nsteps * M * 64);
-        ss.row.resize((size_t)ss.nsteps * ss.R);
-        ss.mult.resize((size_t)ss.nsteps * ss.R);
-        parallel_for(P, [&](int t) {
-            size_t s0 = (size_t)ss.part_begin[t];
-            std::copy(pv[t].begin(), pv[t].end(), ss.val.begin() + s0 * M * 64);
-            std::copy(pc[t].begin(), pc[t].end(), ss.col.begin() + s0 * M * 64);
-            std::copy(pr[t].begin(), pr[t].end(), ss.row.begin() + s0 * ss.R);
-            std::copy(pm[t].begin(), pm[t].end(), ss.mult.begin() + s0 * ss.R);
-        });
+        schedule_partition(pt.k, 0, pt.k - 1, rb, ecol, ev, dg, G, M, pv[t], pc[t], pr[t], pm[t], pn[t]);
+    });
+    ss.part_begin.assign(P + 1, 0);
+    for (int t = 0; t < P; t++) {
+        ss.part_begin[t + 1] = ss.part_begin[t] + pn[t];
+        ss.max_steps = std::max(ss.max_steps, pn[t]);
     }
+    ss.nsteps = ss.part_begin[P];
+    ss.val.resize((size_t)ss.nsteps * M * 64);
+    ss.col.resize((size_t)ss.nsteps * M * 64);
+    ss.row.resize((size_t)ss.nsteps * ss.R);
+    ss.mult.resize((size_t)ss.nsteps * ss.R);
+    parallel_for(P, [&](int t) {
+        size_t s0 = (size_t)ss.part_begin[t];
+        std::copy(pv[t].begin(), pv[t].end(), ss.val.begin() + s0 * M * 64);
+        std::copy(pc[t].begin(), pc[t].end(), ss.col.begin() + s0 * M * 64);
+        std::copy(pr[t].begin(), pr[t].end(), ss.row.begin() + s0 * ss.R);
+        std::copy(pm[t].begin(), pm[t].end(), ss.mult.begin() + s0 * ss.R);
+    });
     return "";
 }
 

@@ -21,9 +21,10 @@
 
 namespace spike {
 
-// One pass of block solves laid out as fixed-shape "steps" executed by one
-// 64-lane wavefront per partition.  A step holds R = 64/G row slots; slot t is
-// served by lanes [t*G, (t+1)*G) and each lane holds M entries ("rounds").
+// Block solves laid out as fixed-shape "steps" executed by one 64-lane
+// wavefront per partition; both solve passes run the same steps.  A step
+// holds R = 64/G row slots; slot t is served by lanes [t*G, (t+1)*G) and each
+// lane holds M entries ("rounds").
 // Within a partition, steps run in order and every row (chunk) in a step only
 // depends on rows finished in earlier steps.  Rows longer than G*M entries are
 // split into chunks; a non-final chunk subtracts its partial dot product from
@@ -43,7 +44,7 @@ struct Part {
     int s = 0, k = 0;       // rows [s, s+k)
     int f = 0;              // first coupling row (local), k if none
     int g = -1;             // last dependent row (local), -1 if none
-    int hi1 = -1;           // pass 1 solves rows [0, hi1]
+    int hi1 = -1;           // pass 1 needs rows [0, hi1] (y at the dependents)
     int kbeg = 0, kcnt = 0; // K_p = kcols[kbeg .. kbeg+kcnt)
     int rows_S = 0;         // spike rows [f, g] when kcnt > 0
     int dbeg = 0, dcnt = 0; // dependents of this partition: dep_rows[dbeg..)
@@ -69,7 +70,7 @@ struct Plan {
     std::vector<int> cpl_ptr;   // CSR over cpl_rows
     std::vector<int> cpl_col;   // reduced index of the coupling column
     std::vector<double> cpl_val;
-    StepStream pass1, pass2;
+    StepStream steps;           // block solve of every partition, rows [0, k)
     int64_t spike_area = 0;
 };
 

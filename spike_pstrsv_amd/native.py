@@ -24,7 +24,8 @@ ERRORS = {1: "SPIKE_ERR_ARG", 2: "SPIKE_ERR_MATRIX", 3: "SPIKE_ERR_DEVICE", 4: "
 # Every symbol declared in include/spike_pstrsv.h.
 EXPORTED = [
     "spike_pstrsv_preproc", "_spike_pstrsv_preproc", "spike_pstrsv", "spike_pstrsv_solve", "_spike_pstrsv_solve",
-    "spike_options_default", "spike_factor_create", "spike_factor_solve", "spike_factor_destroy",
+    "spike_options_default", "spike_factor_create", "spike_factor_solve", "spike_factor_solve_profiled",
+    "spike_factor_destroy",
     "spike_factor_stats", "spike_factor_log", "spike_analyze", "spike_pstrsv_set_partitions",
     "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
 ]
@@ -32,14 +33,15 @@ EXPORTED = [
 
 class Options(C.Structure):
     _fields_ = [("partitions", C.c_int), ("drop_tol", C.c_double), ("precision", C.c_int),
-                ("lanes_per_row", C.c_int), ("verbose", C.c_int)]
+                ("lanes_per_row", C.c_int), ("verbose", C.c_int), ("reduced_solver", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("n", C.c_int), ("partitions", C.c_int), ("nnz", C.c_int64), ("n_reduced", C.c_int),
                 ("nnz_reduced", C.c_int64), ("reduced_levels", C.c_int), ("spike_area", C.c_int64),
                 ("dropped_rowsum_max", C.c_double), ("steps_pass1", C.c_int), ("steps_pass2", C.c_int),
-                ("lanes_per_row", C.c_int), ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double)]
+                ("lanes_per_row", C.c_int), ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double),
+                ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -70,6 +72,8 @@ lib.spike_factor_create.argtypes = [C.POINTER(_vp), _c, _i, _vp, _vp, _vp, C.POI
 lib.spike_factor_create.restype = _i
 lib.spike_factor_solve.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
 lib.spike_factor_solve.restype = _i
+lib.spike_factor_solve_profiled.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp, C.POINTER(_d)]
+lib.spike_factor_solve_profiled.restype = _i
 lib.spike_factor_destroy.argtypes = [_vp]
 lib.spike_factor_destroy.restype = _i
 lib.spike_factor_stats.argtypes = [_vp, C.POINTER(Stats)]
@@ -115,13 +119,18 @@ def _csr(a, ia, ja):
     return a, ia, ja
 
 
-def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False) -> Options:
+REDUCED_SOLVERS = {"auto": 0, "exact": 1, "jacobi": 2}
+
+
+def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False,
+            reduced_solver: str = "auto") -> Options:
     o = Options()
     lib.spike_options_default(C.byref(o))
     o.partitions = partitions
     o.drop_tol = drop_tol
     o.lanes_per_row = lanes_per_row
     o.verbose = int(verbose)
+    o.reduced_solver = REDUCED_SOLVERS[reduced_solver]
     return o
 
 
@@ -129,11 +138,11 @@ class Factor:
     """A preprocessed matrix resident in device memory (spike_factor_t)."""
 
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
-                 lanes_per_row: int = 0, verbose: bool = False):
+                 lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto"):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()
-        o = options(partitions, drop_tol, lanes_per_row, verbose)
+        o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver)
         _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
                                        ja.ctypes.data, C.byref(o)))
         self.handle = h.value
@@ -147,6 +156,15 @@ class Factor:
             x = np.zeros((nrhs, ldx) if nrhs > 1 else ldx, dtype=np.float64)
         _check(lib.spike_factor_solve(self.handle, nrhs, _ptr(b), ldb, _ptr(x), ldx, stream or None))
         return x
+
+    def solve_profiled(self, b, x, nrhs: int = 1, ldb: int | None = None, ldx: int | None = None,
+                       stream: int = 0) -> list[float]:
+        """Solve and return the device time (ms) of pass 1, reduced solve, coupling, pass 2."""
+        ms = (C.c_double * 4)()
+        n = self.n
+        _check(lib.spike_factor_solve_profiled(self.handle, nrhs, _ptr(b), n if ldb is None else ldb, _ptr(x),
+                                               n if ldx is None else ldx, stream or None, ms))
+        return list(ms)
 
     def stats(self) -> dict:
         st = Stats()
