@@ -68,6 +68,7 @@ struct spike_factor_s {
     int nlev = 0;
     bool red_in_lds = false;
     int sweeps = 0;             // > 0: Jacobi sweeps, 0: exact level solve
+    int ring = 16;              // block-solve step records in flight
     double *d_y = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_bc = nullptr;
     // staging for host right-hand sides / solutions
     double *d_stage_b = nullptr, *d_stage_x = nullptr;
@@ -151,6 +152,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     while (P < n && n - (n / P) * (P - 1) > po.kmax_limit) P++;
     po.P = P;
     po.lanes_per_row = f->opts.lanes_per_row;
+    if (const char *e = std::getenv("SPIKE_PSTRSV_RING")) f->ring = std::atoi(e) >= 16 ? 16 : 8;
     std::string err = spike::build_plan(uplo, n, a, ia, ja, po, f->plan);
     if (!err.empty()) return fail(SPIKE_ERR_MATRIX, err);
     Plan &pl = f->plan;
@@ -393,6 +395,7 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
     ba.n = pl.n;
     ba.flip = pl.flip ? 1 : 0;
     ba.kmax = pl.kmax;
+    ba.ring = f->ring;
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));

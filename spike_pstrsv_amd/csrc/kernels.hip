@@ -85,9 +85,11 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     // iterations only (the last < D steps run in the unrolled tail below), so
     // the compiler's memory wait counters stay exact across iterations and
     // loads stay D steps in flight.
+    // Columns and rows stay 16-bit in the ring: widening them right after the
+    // load would make the compiler wait for the load at the loop back-edge.
     double rv[D][M];
-    unsigned rc[D][M];
-    unsigned rrow[D];
+    uint16_t rc[D][M];
+    uint16_t rrow[D];
     double rmul[D];
     if (nst <= 0) goto epilogue;
 #pragma unroll
@@ -95,9 +97,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 #pragma unroll
         for (int m = 0; m < M; m++) {
             rv[d][m] = 0.0;
-            rc[d][m] = (unsigned)k;
+            rc[d][m] = (uint16_t)k;
         }
-        rrow[d] = 0xFFFFu;
+        rrow[d] = (uint16_t)0xFFFFu;
         rmul[d] = 0.0;
     }
     {
@@ -159,10 +161,10 @@ epilogue:
         }
 }
 
-template <int G, int PASS>
-hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
+template <int G, int D, int PASS>
+hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
 {
-    constexpr int M = 1, D = 8;
+    constexpr int M = 1;
     size_t lds = (size_t)(a.kmax + 1) * sizeof(double);
     auto kern = k_block_solve<G, M, D, PASS>;
     if (lds > 65536) {
@@ -171,6 +173,12 @@ hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
     }
     hipLaunchKernelGGL(kern, dim3(P), dim3(64), lds, s, a);
     return hipGetLastError();
+}
+
+template <int G, int PASS>
+hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
+{
+    return a.ring >= 16 ? launch_gd<G, 16, PASS>(P, a, s) : launch_gd<G, 8, PASS>(P, a, s);
 }
 
 template <int PASS>

@@ -37,6 +37,7 @@ struct BlockArgs {
     int n;
     int flip;               // 1: caller's index = n-1-row
     int kmax;               // largest partition (LDS x segment = kmax+1 doubles)
+    int ring;               // step records kept in flight per wavefront (8 or 16)
 };
 
 // Pass 1 (y = D^-1 b, written at the dependents; also the solution of

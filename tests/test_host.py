@@ -1,0 +1,135 @@
+"""Host-side checks (no GPU): the C-ABI library loads and exports every symbol
+of include/spike_pstrsv.h, the symbolic analysis agrees with the oracle and
+with brute force, and invalid input is rejected with an error code."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+from spike_pstrsv_amd import gen
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "spike_pstrsv.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(\w+)\s*\([^;{]*\)\s*;", text)
+    return sorted(set(n for n in names if n not in ("if", "while", "sizeof")))
+
+
+def test_library_exports_every_declared_symbol():
+    from spike_pstrsv_amd import native
+
+    lib = ctypes.CDLL(str(native.LIB_PATH))
+    names = declared_symbols()
+    assert len(names) >= 17
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/spike_pstrsv.h but not exported"
+    assert sorted(native.EXPORTED) == names
+
+
+def test_legacy_signatures_match_reference_header():
+    """The five legacy prototypes keep the reference's argument lists
+    (reference src/spike_pstrsv.h:7, 14, 21, 31, 38)."""
+    text = (ROOT / "include" / "spike_pstrsv.h").read_text()
+    for proto in [
+        "void spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo);",
+        "void _spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo);",
+        "void spike_pstrsv(const char uplo, int m, double *a, MKL_INT *ia, MKL_INT *ja, double *b, double *x, int nthreads);",
+        "void spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthreads);",
+        "void _spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthreads);",
+    ]:
+        assert proto in text
+
+
+@pytest.mark.parametrize("P", [1, 3, 8, 64])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_analyze_matches_oracle(P, uplo):
+    from spike_pstrsv_amd import native
+
+    m = gen.banded(4096, bw=32, q=8, plr=0.05, seed=P)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    st = native.analyze(uplo, m.a, m.ia, m.ja, partitions=P)
+    s = orc.Spike(uplo, m.a, m.ia, m.ja, P)
+    os_ = s.stats()
+    assert st["partitions"] == P
+    assert st["n_reduced"] == os_["n_reduced"]
+    if uplo == "L":
+        assert st["spike_area"] == os_["spike_area"]
+    assert st["steps_pass1"] > 0
+
+
+def test_analyze_lap2d_depth():
+    """2D 5-point lower triangle, 32 grid lines per partition: the block
+    solve's dependency depth is 256 + 32 - 1 levels (anti-diagonals)."""
+    from spike_pstrsv_amd import native
+
+    m = gen.lap2d(256)
+    st = native.analyze("L", m.a, m.ia, m.ja, partitions=8, lanes_per_row=2)
+    assert st["steps_pass1"] == 256 + 32 - 1
+    assert st["n_reduced"] == 7 * 256
+
+
+@pytest.mark.parametrize("bad", ["unsorted", "nodiag", "emptyrow", "ia0", "range"])
+def test_invalid_csr_rejected(bad):
+    from spike_pstrsv_amd import native
+
+    m = gen.banded(100, bw=8, q=4, plr=0.0, seed=5)
+    a, ia, ja = m.a.copy(), m.ia.copy(), m.ja.copy()
+    if bad == "unsorted":
+        j = ia[50]
+        ja[j], ja[j + 1] = ja[j + 1], ja[j]
+    elif bad == "nodiag":
+        ja[ia[50 + 1] - 1] = ja[ia[50 + 1] - 1] - 1 if ia[51] - ia[50] == 1 else 40
+    elif bad == "emptyrow":
+        ia[51:] -= ia[51] - ia[50]
+        ia[51] = ia[50]
+        ia = ia.copy()
+    elif bad == "ia0":
+        ia = ia + 1
+    elif bad == "range":
+        ja[3] = 10_000
+    with pytest.raises(native.SpikeError) as e:
+        native.analyze("L", a, ia, ja, partitions=4)
+    assert e.value.code == 2
+
+
+def test_invalid_arguments():
+    from spike_pstrsv_amd import native
+
+    m = gen.banded(100, seed=6)
+    with pytest.raises(native.SpikeError):
+        native.analyze("X", m.a, m.ia, m.ja, partitions=4)
+    # more partitions than rows: clamped to one row per partition
+    assert native.analyze("L", m.a, m.ia, m.ja, partitions=101)["partitions"] == 100
+
+
+def test_generators_deterministic_and_well_formed():
+    a = gen.banded(5000, seed=9)
+    b = gen.banded(5000, seed=9)
+    assert np.array_equal(a.a, b.a) and np.array_equal(a.ja, b.ja)
+    n = a.n
+    for i in range(0, n, 97):
+        cols = a.ja[a.ia[i]:a.ia[i + 1]]
+        assert cols[-1] == i and np.all(np.diff(cols) > 0)
+        vals = a.a[a.ia[i]:a.ia[i + 1]]
+        assert abs(vals[-1] - (1 + np.abs(vals[:-1]).sum())) < 1e-15
+        band = cols[:-1][cols[:-1] >= max(0, i - 64)]
+        assert len(band) in (min(16, i), min(16, i) - 0) or len(cols) - 1 <= min(16, i) + 1
+    l2 = gen.lap2d(256)
+    assert l2.nnz == 196096 and l2.n == 65536
+    l3 = gen.lap3d(16)
+    assert l3.nnz == 16 ** 3 + 3 * 16 * 16 * 15
+    for m in (l2, l3):
+        assert np.all(m.ja[m.ia[1:] - 1] == np.arange(m.n))
+
+
+def test_banded_1m_expected_size():
+    m = gen.banded(1 << 20)
+    # E[nnz] = 17,846,628 by the generator's definition; realised within 0.1%
+    assert abs(m.nnz - 17_846_628) / 17_846_628 < 1e-3
