@@ -68,8 +68,13 @@ struct spike_factor_s {
     int nlev = 0;
     bool red_in_lds = false;
     int sweeps = 0;             // > 0: Jacobi sweeps, 0: exact level solve
-    int ring = 16;              // block-solve step records in flight
+    std::vector<int> sweep_off; // sweep m: int4 row records off[m-1] .. off[m]
+    int *d_sweep_rows = nullptr;
+    int ring = 8;               // block-solve step records in flight
+    bool cpl_fused = false;     // pass 2 computes its coupling update itself
+    int cmax = 0;               // most coupling entries of one partition
     double *d_y = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_bc = nullptr;
+    double *d_inv_diag = nullptr;
     // staging for host right-hand sides / solutions
     double *d_stage_b = nullptr, *d_stage_x = nullptr;
     size_t stage_cap = 0;
@@ -108,15 +113,20 @@ namespace {
 
 int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps &ds)
 {
-    double *val, *mult;
+    double *val, *mult = nullptr, *late = nullptr;
     uint16_t *col, *row;
     int64_t *pb;
     int rc;
     if ((rc = f->upload(&val, ss.val)) || (rc = f->upload(&col, ss.col)) || (rc = f->upload(&row, ss.row)) ||
-        (rc = f->upload(&mult, ss.mult)) || (rc = f->upload(&pb, ss.part_begin)))
+        (rc = f->upload(&pb, ss.part_begin)))
         return rc;
-    ds.val = val; ds.col = col; ds.row = row; ds.mult = mult; ds.part_begin = pb;
-    ds.G = ss.G; ds.M = ss.M; ds.R = ss.R;
+    if (ss.fwd) {
+        if ((rc = f->upload(&late, ss.late))) return rc;
+    } else if ((rc = f->upload(&mult, ss.mult))) {
+        return rc;
+    }
+    ds.val = val; ds.col = col; ds.row = row; ds.mult = mult; ds.late = late; ds.part_begin = pb;
+    ds.G = ss.G; ds.M = ss.M; ds.R = ss.R; ds.fwd = ss.fwd ? 1 : 0;
     return SPIKE_OK;
 }
 
@@ -133,8 +143,9 @@ void dfree(spike_factor_s *f, void *p, size_t bytes)
 
 int kmax_limit_for_device()
 {
+    // x segment of kmax+2 doubles (zero slot and scratch slot after the rows)
     size_t lds = spike::block_solve_lds_limit();
-    int lim = (int)(lds / sizeof(double)) - 1;
+    int lim = (int)(lds / sizeof(double)) - 2;
     return std::min(lim, 16383);
 }
 
@@ -150,6 +161,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     P = std::min(P, n);
     // raise P until the largest partition fits the LDS x segment
     while (P < n && n - (n / P) * (P - 1) > po.kmax_limit) P++;
+    if (const char *e = std::getenv("SPIKE_PSTRSV_FORWARD")) po.forward = std::atoi(e) != 0;
     po.P = P;
     po.lanes_per_row = f->opts.lanes_per_row;
     if (const char *e = std::getenv("SPIKE_PSTRSV_RING")) f->ring = std::atoi(e) >= 16 ? 16 : 8;
@@ -169,7 +181,12 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         (rc = f->upload(&f->d_cpl_rows, pl.cpl_rows)) || (rc = f->upload(&f->d_cpl_ptr, pl.cpl_ptr)) ||
         (rc = f->upload(&f->d_cpl_col, pl.cpl_col)) || (rc = f->upload(&f->d_cpl_val, pl.cpl_val)))
         return rc;
-    if ((rc = upload_steps(f, pl.steps, f->steps))) return rc;
+    if ((rc = upload_steps(f, pl.steps, f->steps)) || (rc = f->upload(&f->d_inv_diag, pl.inv_diag))) return rc;
+    for (int t = 0; t < P; t++) {
+        const spike::Part &pt = pl.parts[t];
+        f->cmax = std::max(f->cmax, pl.cpl_ptr[pt.cbeg + pt.ccnt] - pl.cpl_ptr[pt.cbeg]);
+    }
+    f->cpl_fused = (size_t)(pl.kmax + 2 + f->cmax) * sizeof(double) <= spike::block_solve_lds_limit();
 
     const int nred = (int)pl.dep_rows.size();
     if ((rc = f->dalloc(&f->d_y, nred)) || (rc = f->dalloc(&f->d_xa, nred)) || (rc = f->dalloc(&f->d_xb, nred)) ||
@@ -293,11 +310,14 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     // terms past its level count vanish exactly.
     int sweeps = -1;
     double jbound = 0.0;
+    std::vector<int> row_sweeps(nred, 0);    // sweeps each row needs
     if (rnnz > 0) {
-        std::vector<double> r(nred, 1.0), r2(nred);
-        std::vector<double> mx;   // mx[j-1] = mx_j
+        std::vector<std::vector<double>> T;  // T[j-1][q] = (|W|^j 1)_q
+        std::vector<double> r(nred, 1.0);
+        std::vector<double> mx;              // mx[j-1] = max_q T[j-1][q]
         bool exact_end = false;
         for (int j = 1; j <= 64; j++) {
+            std::vector<double> r2(nred);
             double m = 0.0;
             for (int q = 0; q < nred; q++) {
                 double acc = 0.0;
@@ -306,16 +326,19 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
                 m = std::max(m, acc);
             }
             mx.push_back(m);
+            T.push_back(r2);
             r.swap(r2);
             if (m == 0.0 || j >= maxlev) { exact_end = true; break; }
             if (m < 1e-20) break;
         }
         const int J = (int)mx.size();
-        double beyond = 0.0;
+        double beyond = 0.0, mass = 0.0;
+        for (int j = 0; j < J; j++) mass += mx[j];
         if (!exact_end) {
             double partial = 1.0;
             for (int j = 0; j < J - 1; j++) partial += mx[j];
             beyond = mx[J - 1] < 1.0 ? mx[J - 1] / (1.0 - mx[J - 1]) * partial : 1e300;
+            mass += beyond;
         }
         // smallest S with sum_{j>S} mx_j <= 1e-16
         double tail = beyond;
@@ -328,6 +351,17 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
                 sweeps = S;
                 jbound = tail;
             }
+            // per row: terms past J are bounded by T_J[q] * mass
+            for (int q = 0; q < nred; q++) {
+                double t = exact_end ? 0.0 : T[J - 1][q] * mass;
+                int m = J;
+                for (int S = J - 1; S >= 0; S--) {
+                    t += T[S][q];
+                    if (t > 1e-16) break;
+                    m = S;
+                }
+                row_sweeps[q] = std::min(m, sweeps);
+            }
         }
     }
     // cost model (microseconds): level solve ~0.15 per level in LDS, ~1 in
@@ -339,11 +373,34 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         return fail(SPIKE_ERR_UNSUPPORTED, "Jacobi sweeps do not reach the reduced-system tolerance in 64 terms");
     if (mode == SPIKE_REDUCED_AUTO) mode = (rnnz > 0 && t_jacobi < t_exact) ? SPIKE_REDUCED_JACOBI : SPIKE_REDUCED_EXACT;
     f->sweeps = (mode == SPIKE_REDUCED_JACOBI && rnnz > 0) ? std::max(sweeps, 1) : 0;
+    if (f->sweeps > 0) {
+        // sweep m computes rows that need >= m sweeps and copies rows that
+        // finished at m-1 (rows finished at 0 are copied in sweeps 1 and 2),
+        // so both buffers hold every finished row's value
+        std::vector<int> lists;
+        f->sweep_off.assign(f->sweeps + 1, 0);
+        for (int m = 1; m <= f->sweeps; m++) {
+            for (int q = 0; q < nred; q++) {
+                const int need = row_sweeps[q];
+                int code = -1;
+                if (need >= m) code = q;
+                else if (need == m - 1 || (need == 0 && m == 2)) code = q | (int)0x80000000;
+                if (code == -1) continue;
+                lists.push_back(code);
+                lists.push_back(hrp[q]);
+                lists.push_back(hrp[q + 1]);
+                lists.push_back(0);
+            }
+            f->sweep_off[m] = (int)(lists.size() / 4);
+        }
+        if ((rc = f->upload(&f->d_sweep_rows, lists))) return rc;
+    }
 
     // host copies no longer needed
     pl.a.clear(); pl.a.shrink_to_fit();
     pl.ja.clear(); pl.ja.shrink_to_fit();
     pl.steps.val.clear(); pl.steps.val.shrink_to_fit(); pl.steps.col.clear(); pl.steps.col.shrink_to_fit();
+    pl.steps.late.clear(); pl.steps.late.shrink_to_fit(); pl.inv_diag.clear(); pl.inv_diag.shrink_to_fit();
     pl.cpl_val.clear(); pl.cpl_val.shrink_to_fit();
 
     spike_stats_t &st = f->stats;
@@ -396,6 +453,13 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
     ba.flip = pl.flip ? 1 : 0;
     ba.kmax = pl.kmax;
     ba.ring = f->ring;
+    ba.inv_diag = f->d_inv_diag;
+    ba.cpl_fused = f->cpl_fused ? 1 : 0;
+    ba.cmax = f->cmax;
+    ba.cpl_ptr = f->d_cpl_ptr;
+    ba.cpl_col = f->d_cpl_col;
+    ba.cpl_val = f->d_cpl_val;
+    ba.xr = f->d_y;
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));
@@ -406,7 +470,9 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
             const double *xin = f->d_y;
             for (int m = 1; m <= f->sweeps; m++) {
                 double *xout = (m & 1) ? f->d_xa : f->d_xb;
-                HIP_TRY(spike::launch_jacobi_sweep(nred, f->d_rrp, f->d_rci, f->d_rv, f->d_y, xin, xout, s));
+                const int o0 = f->sweep_off[m - 1], o1 = f->sweep_off[m];
+                HIP_TRY(spike::launch_jacobi_sweep(o1 - o0, f->d_sweep_rows + 4 * (size_t)o0, f->d_rci, f->d_rv,
+                                                   f->d_y, xin, xout, s));
                 xin = xout;
             }
             xr = xin;
@@ -415,8 +481,10 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
                                                 f->d_rv, f->d_y, f->red_in_lds, s));
         }
         if (ev) HIP_TRY(hipEventRecord(ev[2], s));
-        HIP_TRY(spike::launch_coupling((int)pl.cpl_rows.size(), f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
-                                       f->d_cpl_val, xr, b, f->d_bc, pl.n, ba.flip, s));
+        ba.xr = xr;
+        if (!f->cpl_fused)
+            HIP_TRY(spike::launch_coupling((int)pl.cpl_rows.size(), f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
+                                           f->d_cpl_val, xr, b, f->d_bc, pl.n, ba.flip, s));
         if (ev) HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
     } else if (ev) {

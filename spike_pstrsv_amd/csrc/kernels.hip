@@ -32,6 +32,41 @@ __device__ __forceinline__ double dpp_f64(double v)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// Raw buffer loads: a descriptor per array and partition, a constant per-lane
+// byte offset (VGPR) and a per-step byte offset (SGPR), so refilling the ring
+// needs no vector address arithmetic.  Word 3 0x00020000: raw dword access on
+// gfx9-family parts.  Offsets past num_records read 0.
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ uint16_t buf_u16(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buf_2f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double &a, double &b)
+{
+    u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    u32x2_t lo = {v.x, v.y}, hi = {v.z, v.w};
+    a = __builtin_bit_cast(double, lo);
+    b = __builtin_bit_cast(double, hi);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffull), l);
+    int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Sum over aligned groups of G lanes; every lane of a group gets the same bits.
 template <int G>
 __device__ __forceinline__ double group_sum(double v)
@@ -41,6 +76,42 @@ __device__ __forceinline__ double group_sum(double v)
     if constexpr (G >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror
     if constexpr (G >= 16) v += dpp_f64<0x140>(v); // row_mirror
     return v;
+}
+
+// Pass-2 right-hand side of one partition in LDS: xs[r] = b_r for every row,
+// then b_r - sum_j c_rj * xr[col_j] on the coupling rows (times 1/diag when
+// scaled).  The coupling products are gathered lane-parallel into the LDS
+// scratch prod[] and each coupling row then sums its own products in entry
+// order, so the result does not depend on the lane count.
+__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, double *xs, double *prod,
+                                          bool scaled)
+{
+    const int lane = threadIdx.x, s = pt.s, k = pt.k, n = A.n, flip = A.flip;
+    for (int r = lane; r < k; r += 64) {
+        const int g = s + r;
+        const double bv = A.b[flip ? n - 1 - g : g];
+        xs[r] = scaled ? bv * A.inv_diag[g] : bv;
+    }
+    if (pt.ccnt == 0) return;
+    if (A.cpl_fused) {
+        const int e0 = A.cpl_ptr[pt.cbeg], e1 = A.cpl_ptr[pt.cbeg + pt.ccnt];
+#pragma unroll 4
+        for (int e = e0 + lane; e < e1; e += 64) prod[e - e0] = A.cpl_val[e] * A.xr[A.cpl_col[e]];
+        __syncthreads();
+        for (int c = lane; c < pt.ccnt; c += 64) {
+            const int g = A.cpl_rows[pt.cbeg + c];
+            double acc = 0.0;
+            for (int e = A.cpl_ptr[pt.cbeg + c]; e < A.cpl_ptr[pt.cbeg + c + 1]; e++) acc += prod[e - e0];
+            const double bv = A.b[flip ? n - 1 - g : g] - acc;
+            xs[g - s] = scaled ? bv * A.inv_diag[g] : bv;
+        }
+    } else {
+        __syncthreads();
+        for (int c = lane; c < pt.ccnt; c += 64) {
+            const int g = A.cpl_rows[pt.cbeg + c];
+            xs[g - s] = scaled ? A.bc[pt.cbeg + c] * A.inv_diag[g] : A.bc[pt.cbeg + c];
+        }
+    }
 }
 
 template <int G, int M, int D, int PASS>
@@ -60,23 +131,24 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const int n = A.n, flip = A.flip;
 
     // right-hand side; pass 2 replaces it by b - C x_K on the coupling rows
-    for (int r = lane; r < k; r += 64) {
-        const int g = s + r;
-        xs[r] = A.b[flip ? n - 1 - g : g];
+    if (PASS == 1) {
+        for (int r = lane; r < k; r += 64) {
+            const int g = s + r;
+            xs[r] = A.b[flip ? n - 1 - g : g];
+        }
+    } else {
+        pass2_rhs(A, pt, xs, xs + A.kmax + 2, false);
     }
     if (lane == 0) xs[k] = 0.0;
-    if (PASS == 2) {
-        __syncthreads();
-        for (int c = lane; c < pt.ccnt; c += 64) xs[A.cpl_rows[pt.cbeg + c] - s] = A.bc[pt.cbeg + c];
-    }
     __syncthreads();
 
     const int64_t sb = A.st.part_begin[p];
     const int nst = (int)(A.st.part_begin[p + 1] - sb);
-    const double *__restrict__ sval = A.st.val;
-    const uint16_t *__restrict__ scol = A.st.col;
-    const uint16_t *__restrict__ srow = A.st.row;
-    const double *__restrict__ smul = A.st.mult;
+    const unsigned nstu = (unsigned)(nst > 0 ? nst : 1);
+    const __amdgpu_buffer_rsrc_t bval = make_rsrc(A.st.val + (size_t)sb * M * 64, nstu * M * 512u);
+    const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
+    const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
+    const __amdgpu_buffer_rsrc_t bmul = make_rsrc(A.st.mult + (size_t)sb * R, nstu * R * 8u);
 
     // Register ring of D step records.  It starts as empty padding steps
     // (column k, no row) and the loop starts D steps early, so the first D
@@ -125,16 +197,14 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // Refill slot d with step sidx + D (clamped to the last step), after
         // the last use of the slot's registers.
         auto refill = [&](int d, int sidx) __attribute__((always_inline)) {
-            const int nxt = sidx + D < nst ? sidx + D : nst - 1;
-            const int64_t st = sb + nxt;
+            const unsigned nxt = (unsigned)(sidx + D < nst ? sidx + D : nst - 1);
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const size_t idx = ((size_t)st * M + m) * 64 + lane;
-                rv[d][m] = sval[idx];
-                rc[d][m] = scol[idx];
+                rv[d][m] = buf_f64(bval, (unsigned)(m * 64 + lane) * 8u, nxt * (unsigned)(M * 512));
+                rc[d][m] = buf_u16(bcol, (unsigned)(m * 64 + lane) * 2u, nxt * (unsigned)(M * 128));
             }
-            rrow[d] = srow[st * R + slot];
-            rmul[d] = smul[st * R + slot];
+            rrow[d] = buf_u16(brow, (unsigned)slot * 2u, nxt * (unsigned)(R * 2));
+            rmul[d] = buf_f64(bmul, (unsigned)slot * 8u, nxt * (unsigned)(R * 8));
         };
         int base = -D;
         for (; base + D <= nst; base += D) {
@@ -161,11 +231,124 @@ epilogue:
         }
 }
 
+// Forwarding block solve (16 lanes per row slot, 4 slots per step).  Values
+// are pre-scaled by 1/diag; xs[r] starts as b_r/d_r.  Step s of a partition:
+//   E_s  = xs[row] - sum_early val * xs[col]   (lanes of the slot, DPP sum)
+//   x_s  = E_s - sum_t late[s][slot][t] * x_{s-1}[t]
+// where x_{s-1}[t] is read from the previous step's registers (readlane of
+// lane 16 t).  E_{s+1} only depends on steps <= s-1, so its LDS reads and its
+// reduction are issued during step s and stay off the step-to-step chain,
+// which is readlane + 4 FMAs.  Rows of empty slots are k+1 (scratch).
+template <int D, int PASS>
+__global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
+{
+    extern __shared__ double xs[];
+    constexpr int G = 16, R = 4;
+    const int p = blockIdx.x;
+    const DevPart pt = A.parts[p];
+    const int lane = threadIdx.x;
+    const int slot = lane / G, li = lane % G;
+    const int s = pt.s, k = pt.k;
+    if (PASS == 1 && pt.hi1 < 0) return;
+    if (PASS == 2 && pt.f >= k) return;
+    const int n = A.n, flip = A.flip;
+    const double *__restrict__ invd = A.inv_diag;
+
+    if (PASS == 1) {
+        for (int r = lane; r < k; r += 64) {
+            const int g = s + r;
+            xs[r] = A.b[flip ? n - 1 - g : g] * invd[g];
+        }
+    } else {
+        pass2_rhs(A, pt, xs, xs + A.kmax + 2, true);
+    }
+    if (lane == 0) {
+        xs[k] = 0.0;
+        xs[k + 1] = 0.0;
+    }
+    __syncthreads();
+
+    const int64_t sb = A.st.part_begin[p];
+    const int nst = (int)(A.st.part_begin[p + 1] - sb);
+    if (nst > 0) {
+        const unsigned nstu = (unsigned)nst;
+        const __amdgpu_buffer_rsrc_t bval = make_rsrc(A.st.val + (size_t)sb * 64, nstu * 512u);
+        const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * 64, nstu * 128u);
+        const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
+        const __amdgpu_buffer_rsrc_t blate = make_rsrc(A.st.late + (size_t)sb * R * R, nstu * R * R * 8u);
+        // register ring (16-bit fields stay 16-bit, see k_block_solve)
+        double rv[D];
+        uint16_t rc[D];
+        uint16_t rrow[D];
+        double rl[D][R];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            rv[d] = 0.0;
+            rc[d] = (uint16_t)k;
+            rrow[d] = (uint16_t)(k + 1);
+#pragma unroll
+            for (int t = 0; t < R; t++) rl[d][t] = 0.0;
+        }
+        double E = 0.0, xprev = 0.0;
+        const unsigned ulane = (unsigned)lane, uslot = (unsigned)slot;
+        auto refill = [&](int d, int sidx) __attribute__((always_inline)) {
+            const unsigned nxt = (unsigned)(sidx + D < nst ? sidx + D : nst - 1);
+            rv[d] = buf_f64(bval, ulane * 8u, nxt * 512u);
+            rc[d] = buf_u16(bcol, ulane * 2u, nxt * 128u);
+            rrow[d] = buf_u16(brow, uslot * 2u, nxt * (unsigned)(R * 2));
+            buf_2f64(blate, uslot * 32u, nxt * (unsigned)(R * R * 8), rl[d][0], rl[d][1]);
+            buf_2f64(blate, uslot * 32u + 16u, nxt * (unsigned)(R * R * 8), rl[d][2], rl[d][3]);
+        };
+        // late part of the step in ring slot d, given its early sum
+        auto finish = [&](int d, double e) __attribute__((always_inline)) {
+            double xp[R];
+#pragma unroll
+            for (int t = 0; t < R; t++) xp[t] = readlane_f64(xprev, t * G);
+            double l = rl[d][0] * xp[0];
+#pragma unroll
+            for (int t = 1; t < R; t++) l = fma(rl[d][t], xp[t], l);
+            const double x = e - l;
+            if (li == 0) xs[rrow[d]] = x;
+            xprev = x;
+        };
+        // One step: issue the LDS reads of the next step's early part, run
+        // this step's late part (covers the LDS latency), then reduce the
+        // next step's early part.  Scheduling barriers keep that order.
+        auto pos = [&](int d, bool refill_slot, int sidx) __attribute__((always_inline)) {
+            const int dn = (d + 1) % D;
+            const double xv = xs[rc[dn]];
+            const double bv = xs[li == 0 ? (unsigned)rrow[dn] : (unsigned)k];
+            __builtin_amdgcn_sched_barrier(0);
+            finish(d, E);
+            __builtin_amdgcn_sched_barrier(0);
+            E = group_sum<G>(fma(-rv[dn], xv, bv));
+            if (refill_slot) refill(d, sidx);
+        };
+        int base = -D;
+        for (; base + D <= nst; base += D) {
+#pragma unroll
+            for (int d = 0; d < D; d++) pos(d, true, base + d);
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++)
+            if (base + d < nst) pos(d, false, 0);
+    }
+    __syncthreads();
+
+    if (PASS == 1)
+        for (int q = lane; q < pt.dcnt; q += 64) A.xred[pt.dbeg + q] = xs[A.dep_rows[pt.dbeg + q] - s];
+    if (PASS == 2 || pt.f >= k)
+        for (int r = lane; r < k; r += 64) {
+            const int g = s + r;
+            A.x[flip ? n - 1 - g : g] = xs[r];
+        }
+}
+
 template <int G, int D, int PASS>
 hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
 {
     constexpr int M = 1;
-    size_t lds = (size_t)(a.kmax + 1) * sizeof(double);
+    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * sizeof(double);
     auto kern = k_block_solve<G, M, D, PASS>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -181,9 +364,26 @@ hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
     return a.ring >= 16 ? launch_gd<G, 16, PASS>(P, a, s) : launch_gd<G, 8, PASS>(P, a, s);
 }
 
+template <int D, int PASS>
+hipError_t launch_fwd(int P, const BlockArgs &a, hipStream_t s)
+{
+    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * sizeof(double);
+    auto kern = k_block_solve_fwd<D, PASS>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(P), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int PASS>
 hipError_t launch_pass(int P, const BlockArgs &a, hipStream_t s)
 {
+    if (a.st.fwd) {
+        if (a.st.G != 16) return hipErrorInvalidValue;
+        return a.ring >= 16 ? launch_fwd<16, PASS>(P, a, s) : launch_fwd<8, PASS>(P, a, s);
+    }
     switch (a.st.G) {
     case 1: return launch_g<1, PASS>(P, a, s);
     case 2: return launch_g<2, PASS>(P, a, s);
@@ -224,23 +424,49 @@ __global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, cons
         for (int i = tid; i < nred; i += nt) xred[i] = xl[i];
 }
 
-// One Jacobi sweep of the unit lower reduced system x = y - W x:
-// xout = y - W xin, G lanes per row, fixed summation order.
-template <int G>
-__global__ __launch_bounds__(256) void k_jacobi_sweep(int nred, const int *__restrict__ rp,
+// One Jacobi sweep of the unit lower reduced system x = y - W x over a list
+// of rows: xout[r] = y[r] - (W xin)[r] for the listed rows, G lanes per row,
+// fixed summation order.  Rows of the list flagged with the top bit only copy
+// xin[r] to xout[r] (rows that converged in the previous sweep, so that both
+// buffers hold their final value).  Each lane issues all of its column and
+// value loads, then all of its gathers, before the FMAs.
+template <int G, int U>
+__global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__restrict__ rows,
                                                       const int *__restrict__ ci, const double *__restrict__ v,
                                                       const double *__restrict__ y, const double *__restrict__ xin,
                                                       double *__restrict__ xout)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int row = tid / G, li = tid % G;
+    const int q = tid / G, li = tid % G;
+    int r = -1;
+    bool copy = false;
+    int4 dsc = make_int4(0, 0, 0, 0);
+    if (q < nrows) {
+        dsc = rows[q];                      // {row | copy flag, first entry, end}
+        copy = dsc.x < 0;
+        r = dsc.x & 0x7fffffff;
+    }
     double acc = 0.0;
-    if (row < nred) {
-        const int e = rp[row + 1];
-        for (int j = rp[row] + li; j < e; j += G) acc = fma(v[j], xin[ci[j]], acc);
+    if (r >= 0 && !copy) {
+        const int e = dsc.z;
+        int j = dsc.y + li;
+        for (; j + (U - 1) * G < e; j += U * G) {
+            int c[U];
+            double w[U], xv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                c[u] = ci[j + u * G];
+                w[u] = v[j + u * G];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) xv[u] = xin[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = fma(w[u], xv[u], acc);
+        }
+        for (; j < e; j += G) acc = fma(v[j], xin[ci[j]], acc);
     }
     acc = group_sum<G>(acc);
-    if (li == 0 && row < nred) xout[row] = y[row] - acc;
+    if (li == 0 && r >= 0) xout[r] = copy ? xin[r] : y[r] - acc;
 }
 
 // ---------------------------------------------------------------- coupling
@@ -394,14 +620,14 @@ hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int
     return hipGetLastError();
 }
 
-hipError_t launch_jacobi_sweep(int nred, const int *rp, const int *ci, const double *v, const double *y,
+hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const double *v, const double *y,
                                const double *xin, double *xout, hipStream_t s)
 {
-    if (nred <= 0) return hipSuccess;
-    constexpr int G = 8;
-    const long threads = (long)nred * G;
-    hipLaunchKernelGGL(k_jacobi_sweep<G>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, nred, rp, ci, v,
-                       y, xin, xout);
+    if (nrows <= 0) return hipSuccess;
+    constexpr int G = 8, U = 4;
+    const long threads = (long)nrows * G;
+    hipLaunchKernelGGL((k_jacobi_sweep<G, U>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, nrows,
+                       (const int4 *)rows4, ci, v, y, xin, xout);
     return hipGetLastError();
 }
 

@@ -20,9 +20,11 @@ struct DevSteps {
     const double *val;
     const uint16_t *col;
     const uint16_t *row;
-    const double *mult;
+    const double *mult;          // not fwd
+    const double *late;          // fwd: R x R block per step
     const int64_t *part_begin;   // P+1
     int G, M, R;
+    int fwd;
 };
 
 struct BlockArgs {
@@ -38,6 +40,15 @@ struct BlockArgs {
     int flip;               // 1: caller's index = n-1-row
     int kmax;               // largest partition (LDS x segment = kmax+1 doubles)
     int ring;               // step records kept in flight per wavefront (8 or 16)
+    const double *inv_diag; // fwd: 1/diag per row (lower form)
+    // pass 2 computes b - C x_K itself when cpl_fused (coupling entries of a
+    // partition fit the LDS scratch of cmax doubles); otherwise bc is used
+    int cpl_fused;
+    int cmax;
+    const int *cpl_ptr;
+    const int *cpl_col;
+    const double *cpl_val;
+    const double *xr;       // pass 2: reduced solution
 };
 
 // Pass 1 (y = D^-1 b, written at the dependents; also the solution of
@@ -52,8 +63,10 @@ hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int
                                 const int *rp, const int *ci, const double *v,
                                 double *xred, bool in_lds, hipStream_t s);
 
-// One Jacobi sweep xout = y - W xin of the unit lower reduced system.
-hipError_t launch_jacobi_sweep(int nred, const int *rp, const int *ci, const double *v, const double *y,
+// One Jacobi sweep xout = y - W xin of the unit lower reduced system over a
+// row list of int4 {row, first entry, end entry, 0}.  A row with the top bit
+// set only copies xin to xout.
+hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const double *v, const double *y,
                                const double *xin, double *xout, hipStream_t s);
 
 // bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c.

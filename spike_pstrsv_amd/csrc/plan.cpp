@@ -65,9 +65,10 @@ struct Task {
 // Entries of row r: local columns ecol[rb[r]..rb[r+1]) (ascending, < r).
 void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
                         const std::vector<int> &ecol, const std::vector<double> &evals,
-                        const std::vector<double> &diag, int G, int M,
+                        const std::vector<double> &diag, int G, int M, bool fwd,
                         std::vector<double> &val, std::vector<uint16_t> &col,
-                        std::vector<uint16_t> &row, std::vector<double> &mult, int &nsteps)
+                        std::vector<uint16_t> &row, std::vector<double> &mult,
+                        std::vector<double> &late, int &nsteps)
 {
     const int R = 64 / G, cap = G * M;
     std::vector<Task> tasks;
@@ -110,16 +111,20 @@ void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
     for (int oi = 0; oi < ntask; oi++) {
         int ti = order[oi];
         const Task &t = tasks[ti];
-        bool conflict = used >= R;
-        if (!conflict && prev_chunk[ti] >= 0 && step_of[prev_chunk[ti]] == cur) conflict = true;
-        if (!conflict)
-            for (int e = t.eb; e < t.ee; e++) {
-                int d = ecol[e];
-                if (d >= lo && step_of[final_task[d]] == cur) { conflict = true; break; }
-            }
-        if (conflict) {
-            steps.emplace_back();
-            cur = (int)steps.size() - 1;
+        // earliest step: after every dependency, and after the previous chunk
+        // of the row by >= 1 step (>= 2 in the forwarding layout, whose
+        // accumulator reads run one step ahead)
+        int min_step = 0;
+        if (prev_chunk[ti] >= 0) min_step = step_of[prev_chunk[ti]] + (fwd ? 2 : 1);
+        for (int e = t.eb; e < t.ee; e++) {
+            int d = ecol[e];
+            if (d >= lo) min_step = std::max(min_step, step_of[final_task[d]] + 1);
+        }
+        if (used >= R || cur < min_step) {
+            do {
+                steps.emplace_back();
+                cur = (int)steps.size() - 1;
+            } while (cur < min_step);
             used = 0;
         }
         steps[cur].push_back(ti);
@@ -129,19 +134,49 @@ void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
     nsteps = (int)steps.size();
     val.assign((size_t)nsteps * M * 64, 0.0);
     col.assign((size_t)nsteps * M * 64, (uint16_t)k);
-    row.assign((size_t)nsteps * R, (uint16_t)0xFFFF);
-    mult.assign((size_t)nsteps * R, 0.0);
+    if (!fwd) {
+        row.assign((size_t)nsteps * R, (uint16_t)0xFFFF);
+        mult.assign((size_t)nsteps * R, 0.0);
+        for (int s = 0; s < nsteps; s++) {
+            for (int slot = 0; slot < (int)steps[s].size(); slot++) {
+                const Task &t = tasks[steps[s][slot]];
+                row[(size_t)s * R + slot] = (uint16_t)t.row;
+                mult[(size_t)s * R + slot] = t.final_chunk ? 1.0 / diag[t.row] : 1.0;
+                for (int e = t.eb; e < t.ee; e++) {
+                    int q = e - t.eb;
+                    int m = q / G, li = q % G;
+                    size_t idx = ((size_t)s * M + m) * 64 + slot * G + li;
+                    val[idx] = evals[e];
+                    col[idx] = (uint16_t)ecol[e];
+                }
+            }
+        }
+        return;
+    }
+    // forwarding layout
+    std::vector<int> pos_in_step(ntask, -1);
+    for (int s = 0; s < nsteps; s++)
+        for (int slot = 0; slot < (int)steps[s].size(); slot++) pos_in_step[steps[s][slot]] = slot;
+    row.assign((size_t)nsteps * R, (uint16_t)(k + 1));
+    late.assign((size_t)nsteps * R * R, 0.0);
     for (int s = 0; s < nsteps; s++) {
         for (int slot = 0; slot < (int)steps[s].size(); slot++) {
             const Task &t = tasks[steps[s][slot]];
+            const double inv = 1.0 / diag[t.row];
             row[(size_t)s * R + slot] = (uint16_t)t.row;
-            mult[(size_t)s * R + slot] = t.final_chunk ? 1.0 / diag[t.row] : 1.0;
+            int q = 0;
             for (int e = t.eb; e < t.ee; e++) {
-                int q = e - t.eb;
+                const int d = ecol[e];
+                const int dt = d >= lo ? final_task[d] : -1;
+                if (dt >= 0 && step_of[dt] == s - 1) {
+                    late[((size_t)s * R + slot) * R + pos_in_step[dt]] = evals[e] * inv;
+                    continue;
+                }
                 int m = q / G, li = q % G;
                 size_t idx = ((size_t)s * M + m) * 64 + slot * G + li;
-                val[idx] = evals[e];
-                col[idx] = (uint16_t)ecol[e];
+                val[idx] = evals[e] * inv;
+                col[idx] = (uint16_t)d;
+                q++;
             }
         }
     }
@@ -324,9 +359,12 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     int G = opt.lanes_per_row > 0 ? opt.lanes_per_row : choose_lanes(inblock_len);
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) return "lanes_per_row must be 1, 2, 4, 8 or 16";
     const int M = 1;
+    const bool fwd = opt.forward && G == 16;
     StepStream &ss = pl.steps;
-    ss.G = G; ss.M = M; ss.R = 64 / G;
-    std::vector<std::vector<double>> pv(P), pm(P);
+    ss.G = G; ss.M = M; ss.R = 64 / G; ss.fwd = fwd;
+    pl.inv_diag.resize(n);
+    for (int i = 0; i < n; i++) pl.inv_diag[i] = 1.0 / LV[LA[i + 1] - 1];
+    std::vector<std::vector<double>> pv(P), pm(P), pl_late(P);
     std::vector<std::vector<uint16_t>> pc(P), pr(P);
     std::vector<int> pn(P, 0);
     parallel_for(P, [&](int t) {
@@ -340,7 +378,8 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
             rb[r + 1] = (int)ecol.size();
             dg[r] = LV[LA[i + 1] - 1];
         }
-        schedule_partition(pt.k, 0, pt.k - 1, rb, ecol, ev, dg, G, M, pv[t], pc[t], pr[t], pm[t], pn[t]);
+        schedule_partition(pt.k, 0, pt.k - 1, rb, ecol, ev, dg, G, M, fwd, pv[t], pc[t], pr[t], pm[t], pl_late[t],
+                           pn[t]);
     });
     ss.part_begin.assign(P + 1, 0);
     for (int t = 0; t < P; t++) {
@@ -351,13 +390,15 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     ss.val.resize((size_t)ss.nsteps * M * 64);
     ss.col.resize((size_t)ss.nsteps * M * 64);
     ss.row.resize((size_t)ss.nsteps * ss.R);
-    ss.mult.resize((size_t)ss.nsteps * ss.R);
+    if (fwd) ss.late.resize((size_t)ss.nsteps * ss.R * ss.R);
+    else ss.mult.resize((size_t)ss.nsteps * ss.R);
     parallel_for(P, [&](int t) {
         size_t s0 = (size_t)ss.part_begin[t];
         std::copy(pv[t].begin(), pv[t].end(), ss.val.begin() + s0 * M * 64);
         std::copy(pc[t].begin(), pc[t].end(), ss.col.begin() + s0 * M * 64);
         std::copy(pr[t].begin(), pr[t].end(), ss.row.begin() + s0 * ss.R);
-        std::copy(pm[t].begin(), pm[t].end(), ss.mult.begin() + s0 * ss.R);
+        if (fwd) std::copy(pl_late[t].begin(), pl_late[t].end(), ss.late.begin() + s0 * ss.R * ss.R);
+        else std::copy(pm[t].begin(), pm[t].end(), ss.mult.begin() + s0 * ss.R);
     });
     return "";
 }

@@ -29,15 +29,26 @@ namespace spike {
 // depends on rows finished in earlier steps.  Rows longer than G*M entries are
 // split into chunks; a non-final chunk subtracts its partial dot product from
 // the row's accumulator and only the final chunk scales by 1/diag.
+//
+// Forwarding layout (fwd = true, G = 16, R = 4): every value is pre-scaled by
+// the row's 1/diag (the right-hand side is scaled on load), so a final chunk
+// computes x = acc - sum(val * x_col) with no division.  Entries whose column
+// was produced by the immediately preceding step ("late" entries) are not in
+// the lane layout; they are a dense R x R block late[step][slot][prev_slot]
+// applied to the previous step's results forwarded through registers.  Rows
+// of empty slots are k+1 (a scratch LDS slot).  Consecutive chunks of a long
+// row are at least two steps apart.
 struct StepStream {
     int G = 16, M = 1, R = 4;
+    bool fwd = false;
     int64_t nsteps = 0;
     int max_steps = 0;                  // longest per-partition step chain
     std::vector<int64_t> part_begin;    // P+1 offsets into the step arrays
     std::vector<double> val;            // nsteps * M * 64  [step][round][lane]
     std::vector<uint16_t> col;          // nsteps * M * 64  local column, pad = k_p
     std::vector<uint16_t> row;          // nsteps * R       local row, 0xFFFF = empty
-    std::vector<double> mult;           // nsteps * R       1/diag (final) or 1 (chunk)
+    std::vector<double> mult;           // nsteps * R       1/diag (final) or 1 (chunk); not fwd
+    std::vector<double> late;           // nsteps * R * R   fwd only
 };
 
 struct Part {
@@ -71,6 +82,7 @@ struct Plan {
     std::vector<int> cpl_col;   // reduced index of the coupling column
     std::vector<double> cpl_val;
     StepStream steps;           // block solve of every partition, rows [0, k)
+    std::vector<double> inv_diag;  // n: 1/diag of each row (lower form)
     int64_t spike_area = 0;
 };
 
@@ -78,6 +90,7 @@ struct PlanOptions {
     int P = 1;
     int lanes_per_row = 0;      // 0 = automatic
     int kmax_limit = 16384;     // largest partition a workgroup can hold in LDS
+    bool forward = false;       // use the forwarding layout when G == 16
 };
 
 // Builds the plan.  Returns an empty string on success, else an error message.
