@@ -145,6 +145,7 @@ typedef struct {
     double preproc_seconds;   /* wall time of spike_factor_create              */
     int reduced_sweeps;       /* Jacobi sweeps per solve, 0 = exact solver     */
     double jacobi_bound;      /* bound on the reduced solve's relative error   */
+    int rhs_per_launch;       /* right-hand sides sharing one block solve      */
 } spike_stats_t;
 
 int spike_factor_stats(spike_factor_t f, spike_stats_t *st);

@@ -71,7 +71,9 @@ struct spike_factor_s {
     std::vector<int> sweep_off; // sweep m: int4 row records off[m-1] .. off[m]
     int *d_sweep_rows = nullptr;
     int ring = 8;               // block-solve step records in flight
-    bool cpl_fused = false;     // pass 2 computes its coupling update itself
+    bool cpl_fused = false;     // pass 2 can compute its coupling update itself
+    bool fuse_at[4] = {};       // ... and does so at 1, 2, 4, 8 RHS slots
+    int max_nr = 1;             // right-hand sides per block-solve launch
     int cmax = 0;               // most coupling entries of one partition
     double *d_y = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_bc = nullptr;
     double *d_inv_diag = nullptr;
@@ -149,6 +151,52 @@ int kmax_limit_for_device()
     return std::min(lim, 16383);
 }
 
+// Right-hand sides per block-solve launch.  Every partition is one wavefront
+// that keeps its x segment (kmax+2 doubles per RHS, plus cmax coupling
+// products per RHS when pass 2 fuses its coupling update) in LDS, so the slot
+// count trades work per step against resident partitions per CU.  Relative
+// step cost at 1/2/4/8 slots, measured on gfx950 with 1024-row partitions:
+// 1.0, 1.22, 2.1, 4.2.  A launch costs rounds x step cost, where rounds =
+// ceil(P / (CUs x partitions resident per CU)); the chosen count maximises
+// RHS per unit cost (ties go to fewer slots).  Pass 2 fuses the coupling
+// update only where that keeps its round count.
+void choose_rhs_slots(spike_factor_s *f)
+{
+    const Plan &pl = f->plan;
+    const size_t lim = spike::block_solve_lds_limit();
+    int dev = f->device, ncu = 0, lds_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 1;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        lds_cu <= 0)
+        lds_cu = (int)lim;
+    (void)hipGetLastError();
+    const size_t x_per = (size_t)(pl.kmax + 2) * sizeof(double);
+    const size_t c_per = (size_t)f->cmax * sizeof(double);
+    f->cpl_fused = x_per + c_per <= lim;
+    auto rounds = [&](size_t bytes) -> long {
+        long per_cu = std::max<long>(1, (long)((size_t)lds_cu / std::max<size_t>(bytes, 1)));
+        per_cu = std::min<long>(per_cu, 32);
+        return (pl.P + (long)ncu * per_cu - 1) / ((long)ncu * per_cu);
+    };
+    static const double step_cost[4] = {1.0, 1.22, 2.1, 4.2};
+    double best = -1.0;
+    f->max_nr = 1;
+    for (int li = 0; li < 4; li++) {
+        const int c = 1 << li;
+        if (c > 1 && pl.steps.fwd) break;
+        if (x_per * c > lim) break;
+        const long r1 = rounds(x_per * c);
+        const bool fuse = f->cpl_fused && (x_per + c_per) * c <= lim && rounds((x_per + c_per) * c) == r1;
+        f->fuse_at[li] = fuse;
+        const double t = step_cost[li] * (double)(r1 + r1) + (fuse ? 0.0 : 0.05);
+        const double thr = c / t;
+        if (thr > best * 1.02) {
+            best = thr;
+            f->max_nr = c;
+        }
+    }
+}
+
 int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int *ia, const int *ja)
 {
     const double t0 = now_s();
@@ -182,15 +230,16 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         (rc = f->upload(&f->d_cpl_col, pl.cpl_col)) || (rc = f->upload(&f->d_cpl_val, pl.cpl_val)))
         return rc;
     if ((rc = upload_steps(f, pl.steps, f->steps)) || (rc = f->upload(&f->d_inv_diag, pl.inv_diag))) return rc;
+
+    const int nred = (int)pl.dep_rows.size();
     for (int t = 0; t < P; t++) {
         const spike::Part &pt = pl.parts[t];
         f->cmax = std::max(f->cmax, pl.cpl_ptr[pt.cbeg + pt.ccnt] - pl.cpl_ptr[pt.cbeg]);
     }
-    f->cpl_fused = (size_t)(pl.kmax + 2 + f->cmax) * sizeof(double) <= spike::block_solve_lds_limit();
-
-    const int nred = (int)pl.dep_rows.size();
-    if ((rc = f->dalloc(&f->d_y, nred)) || (rc = f->dalloc(&f->d_xa, nred)) || (rc = f->dalloc(&f->d_xb, nred)) ||
-        (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size())))
+    choose_rhs_slots(f);
+    if ((rc = f->dalloc(&f->d_y, (size_t)nred * f->max_nr)) || (rc = f->dalloc(&f->d_xa, (size_t)nred * f->max_nr)) ||
+        (rc = f->dalloc(&f->d_xb, (size_t)nred * f->max_nr)) ||
+        (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size() * f->max_nr)))
         return rc;
 
     // ---- spikes on the device
@@ -417,6 +466,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.lanes_per_row = pl.steps.G;
     st.reduced_sweeps = f->sweeps;
     st.jacobi_bound = f->sweeps > 0 ? jbound : 0.0;
+    st.rhs_per_launch = f->max_nr;
     st.device_bytes = f->device_bytes;
     st.preproc_seconds = now_s() - t0;
     g_timers[0] += st.preproc_seconds;
@@ -435,11 +485,15 @@ bool is_device_ptr(const void *p)
     return attr.type == hipMemoryTypeDevice || attr.isManaged;
 }
 
-// One right-hand side, all pointers on the device.  ev (optional, 5 events)
-// brackets the phases for the profiling twin.
-int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipEvent_t *ev)
+// nr right-hand sides (<= f->max_nr) in one pass through the kernels, all
+// pointers on the device.  ev (optional, 5 events) brackets the phases for
+// the profiling entry points.
+int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s,
+                hipEvent_t *ev)
 {
     const Plan &pl = f->plan;
+    int slots = 1;
+    while (slots < nr) slots <<= 1;
     spike::BlockArgs ba;
     ba.parts = f->d_parts;
     ba.st = f->steps;
@@ -454,12 +508,17 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
     ba.kmax = pl.kmax;
     ba.ring = f->ring;
     ba.inv_diag = f->d_inv_diag;
-    ba.cpl_fused = f->cpl_fused ? 1 : 0;
+    const bool fuse = f->fuse_at[__builtin_ctz((unsigned)slots)];
+    ba.cpl_fused = fuse ? 1 : 0;
     ba.cmax = f->cmax;
     ba.cpl_ptr = f->d_cpl_ptr;
     ba.cpl_col = f->d_cpl_col;
     ba.cpl_val = f->d_cpl_val;
     ba.xr = f->d_y;
+    ba.nr = nr;
+    ba.nr_slots = slots;
+    ba.ldb = ldb;
+    ba.ldx = ldx;
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));
@@ -472,19 +531,20 @@ int solve_one(spike_factor_s *f, const double *b, double *x, hipStream_t s, hipE
                 double *xout = (m & 1) ? f->d_xa : f->d_xb;
                 const int o0 = f->sweep_off[m - 1], o1 = f->sweep_off[m];
                 HIP_TRY(spike::launch_jacobi_sweep(o1 - o0, f->d_sweep_rows + 4 * (size_t)o0, f->d_rci, f->d_rv,
-                                                   f->d_y, xin, xout, s));
+                                                   f->d_y, xin, xout, slots, s));
                 xin = xout;
             }
             xr = xin;
         } else {
+            const bool in_lds = (size_t)nred * slots * sizeof(double) <= spike::block_solve_lds_limit();
             HIP_TRY(spike::launch_reduced_solve(nred, f->nlev, f->d_levptr, f->d_rlist, f->d_rrp, f->d_rci,
-                                                f->d_rv, f->d_y, f->red_in_lds, s));
+                                                f->d_rv, f->d_y, in_lds, slots, s));
         }
         if (ev) HIP_TRY(hipEventRecord(ev[2], s));
         ba.xr = xr;
-        if (!f->cpl_fused)
+        if (!fuse)
             HIP_TRY(spike::launch_coupling((int)pl.cpl_rows.size(), f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
-                                           f->d_cpl_val, xr, b, f->d_bc, pl.n, ba.flip, s));
+                                           f->d_cpl_val, xr, b, ldb, nr, slots, f->d_bc, pl.n, ba.flip, s));
         if (ev) HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
     } else if (ev) {
@@ -534,8 +594,10 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
     hipEvent_t ev[5];
     if (timed)
         for (int i = 0; i < 5; i++) HIP_TRY(hipEventCreate(&ev[i]));
-    for (int j = 0; j < nrhs; j++) {
-        int rc = solve_one(f, db + (size_t)j * ldb_d, dx + (size_t)j * ldx_d, s, timed ? ev : nullptr);
+    for (int j = 0; j < nrhs; j += f->max_nr) {
+        const int nr = std::min(f->max_nr, nrhs - j);
+        int rc = solve_chunk(f, db + (size_t)j * ldb_d, ldb_d, dx + (size_t)j * ldx_d, ldx_d, nr, s,
+                             timed ? ev : nullptr);
         if (rc) return rc;
         if (timed) {
             HIP_TRY(hipEventSynchronize(ev[4]));

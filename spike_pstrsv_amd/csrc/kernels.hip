@@ -78,43 +78,66 @@ __device__ __forceinline__ double group_sum(double v)
     return v;
 }
 
-// Pass-2 right-hand side of one partition in LDS: xs[r] = b_r for every row,
-// then b_r - sum_j c_rj * xr[col_j] on the coupling rows (times 1/diag when
-// scaled).  The coupling products are gathered lane-parallel into the LDS
-// scratch prod[] and each coupling row then sums its own products in entry
-// order, so the result does not depend on the lane count.
+// Pass-2 right-hand side of one partition in LDS, NR right-hand sides
+// interleaved (xs[i*NR + r]): b for every row, then b - sum_j c_ij xr[col_j]
+// on the coupling rows (times 1/diag when scaled).  The coupling products are
+// gathered lane-parallel into the LDS scratch prod[] and each coupling row
+// then sums its own products in entry order, so the result does not depend
+// on the lane count.  Right-hand sides r >= nr reuse column nr-1.
+template <int NR>
 __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, double *xs, double *prod,
                                           bool scaled)
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k, n = A.n, flip = A.flip;
-    for (int r = lane; r < k; r += 64) {
-        const int g = s + r;
-        const double bv = A.b[flip ? n - 1 - g : g];
-        xs[r] = scaled ? bv * A.inv_diag[g] : bv;
+    for (int i = lane; i < k; i += 64) {
+        const int g = s + i, gm = flip ? n - 1 - g : g;
+        const double sc = scaled ? A.inv_diag[g] : 1.0;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int rr = r < A.nr ? r : A.nr - 1;
+            xs[i * NR + r] = A.b[(size_t)rr * A.ldb + gm] * sc;
+        }
     }
     if (pt.ccnt == 0) return;
     if (A.cpl_fused) {
         const int e0 = A.cpl_ptr[pt.cbeg], e1 = A.cpl_ptr[pt.cbeg + pt.ccnt];
 #pragma unroll 4
-        for (int e = e0 + lane; e < e1; e += 64) prod[e - e0] = A.cpl_val[e] * A.xr[A.cpl_col[e]];
+        for (int e = e0 + lane; e < e1; e += 64) {
+            const double w = A.cpl_val[e];
+            const double *xc = A.xr + (size_t)A.cpl_col[e] * NR;
+#pragma unroll
+            for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * xc[r];
+        }
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
-            const int g = A.cpl_rows[pt.cbeg + c];
-            double acc = 0.0;
-            for (int e = A.cpl_ptr[pt.cbeg + c]; e < A.cpl_ptr[pt.cbeg + c + 1]; e++) acc += prod[e - e0];
-            const double bv = A.b[flip ? n - 1 - g : g] - acc;
-            xs[g - s] = scaled ? bv * A.inv_diag[g] : bv;
+            const int g = A.cpl_rows[pt.cbeg + c], gm = flip ? n - 1 - g : g;
+            const double sc = scaled ? A.inv_diag[g] : 1.0;
+            double acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] = 0.0;
+            for (int e = A.cpl_ptr[pt.cbeg + c]; e < A.cpl_ptr[pt.cbeg + c + 1]; e++)
+#pragma unroll
+                for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int rr = r < A.nr ? r : A.nr - 1;
+                xs[(g - s) * NR + r] = (A.b[(size_t)rr * A.ldb + gm] - acc[r]) * sc;
+            }
         }
     } else {
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
             const int g = A.cpl_rows[pt.cbeg + c];
-            xs[g - s] = scaled ? A.bc[pt.cbeg + c] * A.inv_diag[g] : A.bc[pt.cbeg + c];
+            const double sc = scaled ? A.inv_diag[g] : 1.0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = A.bc[(size_t)(pt.cbeg + c) * NR + r] * sc;
         }
     }
 }
 
-template <int G, int M, int D, int PASS>
+// Block solve with NR right-hand sides sharing each step record; x segments
+// interleaved in LDS as xs[i*NR + r] (i < k+2, slot k holds 0).
+template <int G, int M, int D, int PASS, int NR>
 __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 {
     extern __shared__ double xs[];
@@ -130,16 +153,22 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     if (PASS == 2 && pt.f >= k) return;
     const int n = A.n, flip = A.flip;
 
-    // right-hand side; pass 2 replaces it by b - C x_K on the coupling rows
+    // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
-        for (int r = lane; r < k; r += 64) {
-            const int g = s + r;
-            xs[r] = A.b[flip ? n - 1 - g : g];
+        for (int i = lane; i < k; i += 64) {
+            const int g = s + i, gm = flip ? n - 1 - g : g;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int rr = r < A.nr ? r : A.nr - 1;
+                xs[i * NR + r] = A.b[(size_t)rr * A.ldb + gm];
+            }
         }
     } else {
-        pass2_rhs(A, pt, xs, xs + A.kmax + 2, false);
+        pass2_rhs<NR>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false);
     }
-    if (lane == 0) xs[k] = 0.0;
+    if (lane == 0)
+#pragma unroll
+        for (int r = 0; r < NR; r++) xs[k * NR + r] = 0.0;
     __syncthreads();
 
     const int64_t sb = A.st.part_begin[p];
@@ -149,7 +178,6 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
     const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
     const __amdgpu_buffer_rsrc_t bmul = make_rsrc(A.st.mult + (size_t)sb * R, nstu * R * 8u);
-
     // Register ring of D step records.  It starts as empty padding steps
     // (column k, no row) and the loop starts D steps early, so the first D
     // iterations compute nothing and only issue loads.  Every load then comes
@@ -176,23 +204,31 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     }
     {
         // One step from ring slot d: LDS reads first (operands and the row's
-        // accumulator), then the dot product with the accumulator folded into
+        // accumulator), then the dot products with the accumulator folded into
         // lane 0 of the group, so the accumulator read stays ahead of the
         // reduction.
         auto step = [&](int d) __attribute__((always_inline)) {
             const unsigned row = rrow[d];
             const bool valid = row != 0xFFFFu;
-            double xv[M];
+            const unsigned arow = valid ? row : (unsigned)k;
+            double xv[M][NR], bv[NR], t[NR];
 #pragma unroll
-            for (int m = 0; m < M; m++) xv[m] = xs[rc[d][m]];
-            const double bv = xs[valid ? row : (unsigned)k];
-            double t = 0.0;
+            for (int m = 0; m < M; m++)
 #pragma unroll
-            for (int m = 0; m < M; m++) t = fma(-rv[d][m], xv[m], t);
-            t = li == 0 ? t + bv : t;
-            t = group_sum<G>(t);
-            const double res = t * rmul[d];
-            if (li == 0 && valid) xs[row] = res;
+                for (int r = 0; r < NR; r++) xv[m][r] = xs[(unsigned)rc[d][m] * NR + r];
+#pragma unroll
+            for (int r = 0; r < NR; r++) bv[r] = xs[arow * NR + r];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                t[r] = 0.0;
+#pragma unroll
+                for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
+                t[r] = li == 0 ? t[r] + bv[r] : t[r];
+                t[r] = group_sum<G>(t[r]);
+            }
+            if (li == 0 && valid)
+#pragma unroll
+                for (int r = 0; r < NR; r++) xs[row * NR + r] = t[r] * rmul[d];
         };
         // Refill slot d with step sidx + D (clamped to the last step), after
         // the last use of the slot's registers.
@@ -222,12 +258,18 @@ epilogue:
     __syncthreads();
 
     if (PASS == 1)
-        for (int q = lane; q < pt.dcnt; q += 64) A.xred[pt.dbeg + q] = xs[A.dep_rows[pt.dbeg + q] - s];
+        for (int q = lane; q < pt.dcnt; q += 64) {
+            const int i = A.dep_rows[pt.dbeg + q] - s;
+#pragma unroll
+            for (int r = 0; r < NR; r++) A.xred[(size_t)(pt.dbeg + q) * NR + r] = xs[i * NR + r];
+        }
     // pass 2 writes the solution; without coupling, pass 1's result is final
     if (PASS == 2 || pt.f >= k)
-        for (int r = lane; r < k; r += 64) {
-            const int g = s + r;
-            A.x[flip ? n - 1 - g : g] = xs[r];
+        for (int i = lane; i < k; i += 64) {
+            const int g = s + i, gm = flip ? n - 1 - g : g;
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                if (r < A.nr) A.x[(size_t)r * A.ldx + gm] = xs[i * NR + r];
         }
 }
 
@@ -260,7 +302,7 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
             xs[r] = A.b[flip ? n - 1 - g : g] * invd[g];
         }
     } else {
-        pass2_rhs(A, pt, xs, xs + A.kmax + 2, true);
+        pass2_rhs<1>(A, pt, xs, xs + A.kmax + 2, true);
     }
     if (lane == 0) {
         xs[k] = 0.0;
@@ -344,12 +386,12 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
         }
 }
 
-template <int G, int D, int PASS>
+template <int G, int D, int PASS, int NR>
 hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
 {
     constexpr int M = 1;
-    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * sizeof(double);
-    auto kern = k_block_solve<G, M, D, PASS>;
+    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * NR * sizeof(double);
+    auto kern = k_block_solve<G, M, D, PASS, NR>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -361,7 +403,13 @@ hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
 template <int G, int PASS>
 hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
 {
-    return a.ring >= 16 ? launch_gd<G, 16, PASS>(P, a, s) : launch_gd<G, 8, PASS>(P, a, s);
+    switch (a.nr_slots) {
+    case 1: return launch_gd<G, 8, PASS, 1>(P, a, s);
+    case 2: return launch_gd<G, 8, PASS, 2>(P, a, s);
+    case 4: return launch_gd<G, 8, PASS, 4>(P, a, s);
+    case 8: return launch_gd<G, 8, PASS, 8>(P, a, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int D, int PASS>
@@ -381,7 +429,7 @@ template <int PASS>
 hipError_t launch_pass(int P, const BlockArgs &a, hipStream_t s)
 {
     if (a.st.fwd) {
-        if (a.st.G != 16) return hipErrorInvalidValue;
+        if (a.st.G != 16 || a.nr_slots != 1) return hipErrorInvalidValue;
         return a.ring >= 16 ? launch_fwd<16, PASS>(P, a, s) : launch_fwd<8, PASS>(P, a, s);
     }
     switch (a.st.G) {
@@ -396,7 +444,9 @@ hipError_t launch_pass(int P, const BlockArgs &a, hipStream_t s)
 
 // ---------------------------------------------------------------- reduced
 
-template <bool IN_LDS>
+// Exact forward substitution of the unit lower reduced system, one workgroup,
+// level by level; NR right-hand sides interleaved (X[q*NR + r]).
+template <bool IN_LDS, int NR>
 __global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, const int *__restrict__ levptr,
                                                         const int *__restrict__ rlist, const int *__restrict__ rp,
                                                         const int *__restrict__ ci, const double *__restrict__ v,
@@ -406,22 +456,30 @@ __global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, cons
     const int tid = threadIdx.x, nt = blockDim.x;
     double *X = xred;
     if (IN_LDS) {
-        for (int i = tid; i < nred; i += nt) xl[i] = xred[i];
+        for (int i = tid; i < nred * NR; i += nt) xl[i] = xred[i];
         __syncthreads();
         X = xl;
     }
     for (int L = 0; L < nlev; L++) {
         const int b = levptr[L], e = levptr[L + 1];
         for (int idx = b + tid; idx < e; idx += nt) {
-            const int r = rlist[idx];
-            double acc = 0.0;
-            for (int j = rp[r]; j < rp[r + 1]; j++) acc = fma(v[j], X[ci[j]], acc);
-            X[r] -= acc;
+            const int q = rlist[idx];
+            double acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] = 0.0;
+            for (int j = rp[q]; j < rp[q + 1]; j++) {
+                const double w = v[j];
+                const int c = ci[j];
+#pragma unroll
+                for (int r = 0; r < NR; r++) acc[r] = fma(w, X[c * NR + r], acc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) X[q * NR + r] -= acc[r];
         }
         __syncthreads();
     }
     if (IN_LDS)
-        for (int i = tid; i < nred; i += nt) xred[i] = xl[i];
+        for (int i = tid; i < nred * NR; i += nt) xred[i] = xl[i];
 }
 
 // One Jacobi sweep of the unit lower reduced system x = y - W x over a list
@@ -430,7 +488,7 @@ __global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, cons
 // xin[r] to xout[r] (rows that converged in the previous sweep, so that both
 // buffers hold their final value).  Each lane issues all of its column and
 // value loads, then all of its gathers, before the FMAs.
-template <int G, int U>
+template <int G, int U, int NR>
 __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__restrict__ rows,
                                                       const int *__restrict__ ci, const double *__restrict__ v,
                                                       const double *__restrict__ y, const double *__restrict__ xin,
@@ -446,43 +504,70 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
         copy = dsc.x < 0;
         r = dsc.x & 0x7fffffff;
     }
-    double acc = 0.0;
+    double acc[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = 0.0;
     if (r >= 0 && !copy) {
         const int e = dsc.z;
         int j = dsc.y + li;
         for (; j + (U - 1) * G < e; j += U * G) {
             int c[U];
-            double w[U], xv[U];
+            double w[U], xv[U][NR];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 c[u] = ci[j + u * G];
                 w[u] = v[j + u * G];
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) xv[u] = xin[c[u]];
+            for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int u = 0; u < U; u++) acc = fma(w[u], xv[u], acc);
+                for (int t = 0; t < NR; t++) xv[u][t] = xin[(size_t)c[u] * NR + t];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int t = 0; t < NR; t++) acc[t] = fma(w[u], xv[u][t], acc[t]);
         }
-        for (; j < e; j += G) acc = fma(v[j], xin[ci[j]], acc);
+        for (; j < e; j += G) {
+            const double w = v[j];
+            const size_t c = (size_t)ci[j] * NR;
+#pragma unroll
+            for (int t = 0; t < NR; t++) acc[t] = fma(w, xin[c + t], acc[t]);
+        }
     }
-    acc = group_sum<G>(acc);
-    if (li == 0 && r >= 0) xout[r] = copy ? xin[r] : y[r] - acc;
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
+    if (li == 0 && r >= 0)
+#pragma unroll
+        for (int t = 0; t < NR; t++)
+            xout[(size_t)r * NR + t] = copy ? xin[(size_t)r * NR + t] : y[(size_t)r * NR + t] - acc[t];
 }
 
 // ---------------------------------------------------------------- coupling
 
+template <int NR>
 __global__ __launch_bounds__(256) void k_coupling(int ncpl, const int *__restrict__ rows,
                                                   const int *__restrict__ ptr, const int *__restrict__ col,
                                                   const double *__restrict__ val, const double *__restrict__ xred,
-                                                  const double *__restrict__ b, double *__restrict__ bc, int n,
-                                                  int flip)
+                                                  const double *__restrict__ b, int ldb, int nr,
+                                                  double *__restrict__ bc, int n, int flip)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncpl) return;
-    const int g = rows[c];
-    double acc = 0.0;
-    for (int j = ptr[c]; j < ptr[c + 1]; j++) acc = fma(val[j], xred[col[j]], acc);
-    bc[c] = b[flip ? n - 1 - g : g] - acc;
+    const int g = rows[c], gm = flip ? n - 1 - g : g;
+    double acc[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = 0.0;
+    for (int j = ptr[c]; j < ptr[c + 1]; j++) {
+        const double w = val[j];
+        const size_t x0 = (size_t)col[j] * NR;
+#pragma unroll
+        for (int t = 0; t < NR; t++) acc[t] = fma(w, xred[x0 + t], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+        const int tt = t < nr ? t : nr - 1;
+        bc[(size_t)c * NR + t] = b[(size_t)tt * ldb + gm] - acc[t];
+    }
 }
 
 // ---------------------------------------------------------------- spikes
@@ -601,43 +686,69 @@ hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s
     return pass == 1 ? launch_pass<1>(P, a, s) : launch_pass<2>(P, a, s);
 }
 
-hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int *rlist, const int *rp,
-                                const int *ci, const double *v, double *xred, bool in_lds, hipStream_t s)
+template <bool IN_LDS, int NR>
+hipError_t launch_reduced_nr(int nred, int nlev, const int *levptr, const int *rlist, const int *rp, const int *ci,
+                             const double *v, double *xred, hipStream_t s)
 {
-    if (nlev <= 0) return hipSuccess;
-    if (in_lds) {
-        size_t lds = (size_t)nred * sizeof(double);
-        auto kern = k_reduced_solve<true>;
-        if (lds > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL(kern, dim3(1), dim3(1024), lds, s, nred, nlev, levptr, rlist, rp, ci, v, xred);
-    } else {
-        hipLaunchKernelGGL(k_reduced_solve<false>, dim3(1), dim3(1024), 0, s, nred, nlev, levptr, rlist, rp, ci, v,
-                           xred);
+    size_t lds = IN_LDS ? (size_t)nred * NR * sizeof(double) : 0;
+    auto kern = k_reduced_solve<IN_LDS, NR>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
     }
+    hipLaunchKernelGGL(kern, dim3(1), dim3(1024), lds, s, nred, nlev, levptr, rlist, rp, ci, v, xred);
     return hipGetLastError();
 }
 
+hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int *rlist, const int *rp,
+                                const int *ci, const double *v, double *xred, bool in_lds, int nr_slots,
+                                hipStream_t s)
+{
+    if (nlev <= 0) return hipSuccess;
+#define SPIKE_RED(NR)                                                                                    \
+    return in_lds ? launch_reduced_nr<true, NR>(nred, nlev, levptr, rlist, rp, ci, v, xred, s)           \
+                  : launch_reduced_nr<false, NR>(nred, nlev, levptr, rlist, rp, ci, v, xred, s)
+    switch (nr_slots) {
+    case 1: SPIKE_RED(1);
+    case 2: SPIKE_RED(2);
+    case 4: SPIKE_RED(4);
+    case 8: SPIKE_RED(8);
+    default: return hipErrorInvalidValue;
+    }
+#undef SPIKE_RED
+}
+
 hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const double *v, const double *y,
-                               const double *xin, double *xout, hipStream_t s)
+                               const double *xin, double *xout, int nr_slots, hipStream_t s)
 {
     if (nrows <= 0) return hipSuccess;
     constexpr int G = 8, U = 4;
     const long threads = (long)nrows * G;
-    hipLaunchKernelGGL((k_jacobi_sweep<G, U>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, nrows,
-                       (const int4 *)rows4, ci, v, y, xin, xout);
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    const int4 *r4 = (const int4 *)rows4;
+    switch (nr_slots) {
+    case 1: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 1>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
+    case 2: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 2>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
+    case 4: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 4>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
+    case 8: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 8>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *cpl_val, const double *xred, const double *b, double *bc, int n, int flip,
-                           hipStream_t s)
+                           const double *cpl_val, const double *xred, const double *b, int ldb, int nr,
+                           int nr_slots, double *bc, int n, int flip, hipStream_t s)
 {
     if (ncpl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_coupling, dim3((ncpl + 255) / 256), dim3(256), 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col,
-                       cpl_val, xred, b, bc, n, flip);
+    const dim3 grid((ncpl + 255) / 256), block(256);
+    switch (nr_slots) {
+    case 1: hipLaunchKernelGGL(k_coupling<1>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
+    case 2: hipLaunchKernelGGL(k_coupling<2>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
+    case 4: hipLaunchKernelGGL(k_coupling<4>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
+    case 8: hipLaunchKernelGGL(k_coupling<8>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

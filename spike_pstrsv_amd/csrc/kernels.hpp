@@ -49,6 +49,11 @@ struct BlockArgs {
     const int *cpl_col;
     const double *cpl_val;
     const double *xr;       // pass 2: reduced solution
+    // right-hand sides of this launch: nr active (<= nr_slots) columns with
+    // leading dimensions ldb / ldx; reduced vectors (xred, xr, bc) hold
+    // nr_slots interleaved values per index
+    int nr, nr_slots;
+    int ldb, ldx;
 };
 
 // Pass 1 (y = D^-1 b, written at the dependents; also the solution of
@@ -61,18 +66,19 @@ hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s
 // levptr[nlev+1] (device) delimits the levels.
 hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int *rlist,
                                 const int *rp, const int *ci, const double *v,
-                                double *xred, bool in_lds, hipStream_t s);
+                                double *xred, bool in_lds, int nr_slots, hipStream_t s);
 
 // One Jacobi sweep xout = y - W xin of the unit lower reduced system over a
 // row list of int4 {row, first entry, end entry, 0}.  A row with the top bit
 // set only copies xin to xout.
 hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const double *v, const double *y,
-                               const double *xin, double *xout, hipStream_t s);
+                               const double *xin, double *xout, int nr_slots, hipStream_t s);
 
-// bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c.
+// bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c and
+// right-hand side (bc, xred interleaved with nr_slots values per index).
 hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *cpl_val, const double *xred, const double *b,
-                           double *bc, int n, int flip, hipStream_t s);
+                           const double *cpl_val, const double *xred, const double *b, int ldb, int nr,
+                           int nr_slots, double *bc, int n, int flip, hipStream_t s);
 
 // Spike build: for every partition with kcnt > 0, W_p = D0^-1 C[:, K_p] on
 // rows [f, g] (row-major, ld = kcnt) using the lower-form CSR (ia, ja, a).

@@ -41,7 +41,8 @@ class Stats(C.Structure):
                 ("nnz_reduced", C.c_int64), ("reduced_levels", C.c_int), ("spike_area", C.c_int64),
                 ("dropped_rowsum_max", C.c_double), ("steps_pass1", C.c_int), ("steps_pass2", C.c_int),
                 ("lanes_per_row", C.c_int), ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double),
-                ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double)]
+                ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double),
+                ("rhs_per_launch", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -100,6 +100,37 @@ def test_legacy_api_manufactured(native, lap2d):
     assert native.current_stats()["partitions"] == 8
 
 
+@pytest.mark.parametrize("nrhs", [2, 3, 5, 8, 13])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_multi_rhs(native, nrhs, uplo):
+    """Right-hand sides share each block-solve launch (up to 8 per chunk)."""
+    m = gen.banded(1 << 15, seed=gen.SEED_BASE + 70 + nrhs)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    B = gen.rhs(m.n, nrhs, seed=gen.SEED_BASE + 71)
+    f = native.Factor(uplo, m.a, m.ia, m.ja, partitions=32)
+    X = f.solve(B, nrhs=nrhs)
+    for j in range(nrhs):
+        assert relerr(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])) <= TOL
+    # column j solved alone gives the same bits
+    xj = f.solve(B[nrhs - 1].copy())
+    assert np.array_equal(xj, X[nrhs - 1])
+
+
+def test_multi_rhs_leading_dimension(native):
+    m = gen.lap2d(128)
+    n = m.n
+    ld = n + 37
+    B = np.zeros((4, ld))
+    B[:, :n] = gen.rhs(n, 4, seed=gen.SEED_BASE + 75)
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=8)
+    X = np.full((4, ld), 7.0)
+    f.solve(B, X, nrhs=4, ldb=ld, ldx=ld)
+    for j in range(4):
+        assert relerr(X[j, :n], orc.ldsol(m.a, m.ia, m.ja, B[j, :n])) <= TOL
+        assert np.all(X[j, n:] == 7.0), "entries past n must not be written"
+
+
 def test_multi_rhs_and_device_pointers(native):
     import torch
 
