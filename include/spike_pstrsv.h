@@ -78,7 +78,11 @@ enum {
 
 /* Precision modes of a factor. */
 enum {
-    SPIKE_PREC_FP64 = 0       /* FP64 values and arithmetic throughout         */
+    SPIKE_PREC_FP64 = 0,      /* FP64 values and arithmetic throughout         */
+    SPIKE_PREC_FP32_REFINED = 1 /* block solves and coupling updates with FP32 */
+                              /* values and FP32 accumulation; the residual    */
+                              /* b - A x, the reduced system and the update    */
+                              /* x += d stay FP64 (iterative refinement)       */
 };
 
 typedef struct spike_factor_s *spike_factor_t;
@@ -94,6 +98,11 @@ typedef struct {
                               /* 0 = automatic                                 */
     int verbose;              /* print the partition log after preprocessing   */
     int reduced_solver;       /* SPIKE_REDUCED_*                               */
+    int refine_steps;         /* SPIKE_PREC_FP32_REFINED: refinement steps per */
+                              /* solve; < 0 = calibrated at preprocessing on a */
+                              /* random right-hand side until the predicted    */
+                              /* relative error is <= refine_tol               */
+    double refine_tol;        /* target of the calibration; <= 0 = 1e-14       */
 } spike_options_t;
 
 /* Reduced-system solvers. */
@@ -146,6 +155,10 @@ typedef struct {
     int reduced_sweeps;       /* Jacobi sweeps per solve, 0 = exact solver     */
     double jacobi_bound;      /* bound on the reduced solve's relative error   */
     int rhs_per_launch;       /* right-hand sides sharing one block solve      */
+    int precision;            /* SPIKE_PREC_* of the factor                    */
+    int refine_steps;         /* refinement steps per solve (FP32_REFINED)     */
+    double refine_rate;       /* calibrated contraction per refinement step    */
+    double refine_est_err;    /* calibrated predicted relative error           */
 } spike_stats_t;
 
 int spike_factor_stats(spike_factor_t f, spike_stats_t *st);

@@ -80,6 +80,15 @@ struct spike_factor_s {
     // staging for host right-hand sides / solutions
     double *d_stage_b = nullptr, *d_stage_x = nullptr;
     size_t stage_cap = 0;
+    // SPIKE_PREC_FP32_REFINED: FP32 step and coupling values, the FP64
+    // lower-form CSR for the residual, a residual workspace (n x max_nr)
+    bool fp32 = false;
+    float *d_cpl_val32 = nullptr;
+    const double *d_a = nullptr;
+    const int *d_ia = nullptr, *d_ja = nullptr;
+    double *d_r = nullptr;
+    int refine_steps = 0;
+    double refine_rate = 0.0, refine_est = 0.0;
 
     template <class T>
     int dalloc(T **p, size_t count)
@@ -113,21 +122,35 @@ struct spike_factor_s {
 
 namespace {
 
-int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps &ds)
+std::vector<float> to_f32(const std::vector<double> &v)
 {
-    double *val, *mult = nullptr, *late = nullptr;
+    return std::vector<float>(v.begin(), v.end());
+}
+
+// Step stream to the device: FP64 values and multipliers, or (fp32) their
+// FP32 roundings only.
+int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps &ds, bool fp32)
+{
+    double *val = nullptr, *mult = nullptr, *late = nullptr;
+    float *val32 = nullptr, *mult32 = nullptr;
     uint16_t *col, *row;
     int64_t *pb;
     int rc;
-    if ((rc = f->upload(&val, ss.val)) || (rc = f->upload(&col, ss.col)) || (rc = f->upload(&row, ss.row)) ||
-        (rc = f->upload(&pb, ss.part_begin)))
+    if ((rc = f->upload(&col, ss.col)) || (rc = f->upload(&row, ss.row)) || (rc = f->upload(&pb, ss.part_begin)))
         return rc;
-    if (ss.fwd) {
-        if ((rc = f->upload(&late, ss.late))) return rc;
-    } else if ((rc = f->upload(&mult, ss.mult))) {
-        return rc;
+    if (fp32) {
+        if (ss.fwd) return fail(SPIKE_ERR_UNSUPPORTED, "the forwarding block solve has no FP32 form");
+        if ((rc = f->upload(&val32, to_f32(ss.val))) || (rc = f->upload(&mult32, to_f32(ss.mult)))) return rc;
+    } else {
+        if ((rc = f->upload(&val, ss.val))) return rc;
+        if (ss.fwd) {
+            if ((rc = f->upload(&late, ss.late))) return rc;
+        } else if ((rc = f->upload(&mult, ss.mult))) {
+            return rc;
+        }
     }
     ds.val = val; ds.col = col; ds.row = row; ds.mult = mult; ds.late = late; ds.part_begin = pb;
+    ds.val32 = val32; ds.mult32 = mult32;
     ds.G = ss.G; ds.M = ss.M; ds.R = ss.R; ds.fwd = ss.fwd ? 1 : 0;
     return SPIKE_OK;
 }
@@ -170,8 +193,9 @@ void choose_rhs_slots(spike_factor_s *f)
         lds_cu <= 0)
         lds_cu = (int)lim;
     (void)hipGetLastError();
-    const size_t x_per = (size_t)(pl.kmax + 2) * sizeof(double);
-    const size_t c_per = (size_t)f->cmax * sizeof(double);
+    const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
+    const size_t x_per = (size_t)(pl.kmax + 2) * vb;
+    const size_t c_per = (size_t)f->cmax * vb;
     f->cpl_fused = x_per + c_per <= lim;
     auto rounds = [&](size_t bytes) -> long {
         long per_cu = std::max<long>(1, (long)((size_t)lds_cu / std::max<size_t>(bytes, 1)));
@@ -197,6 +221,8 @@ void choose_rhs_slots(spike_factor_s *f)
     }
 }
 
+int calibrate_refinement(spike_factor_s *f);
+
 int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int *ia, const int *ja)
 {
     const double t0 = now_s();
@@ -209,7 +235,8 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     P = std::min(P, n);
     // raise P until the largest partition fits the LDS x segment
     while (P < n && n - (n / P) * (P - 1) > po.kmax_limit) P++;
-    if (const char *e = std::getenv("SPIKE_PSTRSV_FORWARD")) po.forward = std::atoi(e) != 0;
+    f->fp32 = f->opts.precision == SPIKE_PREC_FP32_REFINED;
+    if (const char *e = std::getenv("SPIKE_PSTRSV_FORWARD")) po.forward = std::atoi(e) != 0 && !f->fp32;
     po.P = P;
     po.lanes_per_row = f->opts.lanes_per_row;
     if (const char *e = std::getenv("SPIKE_PSTRSV_RING")) f->ring = std::atoi(e) >= 16 ? 16 : 8;
@@ -229,7 +256,9 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         (rc = f->upload(&f->d_cpl_rows, pl.cpl_rows)) || (rc = f->upload(&f->d_cpl_ptr, pl.cpl_ptr)) ||
         (rc = f->upload(&f->d_cpl_col, pl.cpl_col)) || (rc = f->upload(&f->d_cpl_val, pl.cpl_val)))
         return rc;
-    if ((rc = upload_steps(f, pl.steps, f->steps)) || (rc = f->upload(&f->d_inv_diag, pl.inv_diag))) return rc;
+    if ((rc = upload_steps(f, pl.steps, f->steps, f->fp32)) || (rc = f->upload(&f->d_inv_diag, pl.inv_diag)))
+        return rc;
+    if (f->fp32 && (rc = f->upload(&f->d_cpl_val32, to_f32(pl.cpl_val)))) return rc;
 
     const int nred = (int)pl.dep_rows.size();
     for (int t = 0; t < P; t++) {
@@ -241,6 +270,21 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         (rc = f->dalloc(&f->d_xb, (size_t)nred * f->max_nr)) ||
         (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size() * f->max_nr)))
         return rc;
+    if (f->fp32 && (rc = f->dalloc(&f->d_r, (size_t)n * f->max_nr))) return rc;
+
+    // lower-form CSR on the device: the spike build reads it, the FP32 mode
+    // keeps it for the FP64 residual
+    int *d_ia = nullptr, *d_ja = nullptr;
+    double *d_a = nullptr;
+    if (nred > 0 || f->fp32) {
+        if ((rc = f->upload(&d_ia, pl.ia)) || (rc = f->upload(&d_ja, pl.ja)) || (rc = f->upload(&d_a, pl.a)))
+            return rc;
+        if (f->fp32) {
+            f->d_a = d_a;
+            f->d_ia = d_ia;
+            f->d_ja = d_ja;
+        }
+    }
 
     // ---- spikes on the device
     std::vector<spike::SpikePart> sp;
@@ -262,10 +306,9 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     double t2 = now_s(), t3 = t2;
     if (nred > 0) {
         spike::SpikePart *d_sp;
-        int *d_kcols, *d_ia, *d_ja;
-        double *d_a, *d_W;
-        if ((rc = f->upload(&d_sp, sp)) || (rc = f->upload(&d_kcols, pl.kcols)) || (rc = f->upload(&d_ia, pl.ia)) ||
-            (rc = f->upload(&d_ja, pl.ja)) || (rc = f->upload(&d_a, pl.a)) || (rc = f->dalloc(&d_W, (size_t)area)))
+        int *d_kcols;
+        double *d_W;
+        if ((rc = f->upload(&d_sp, sp)) || (rc = f->upload(&d_kcols, pl.kcols)) || (rc = f->dalloc(&d_W, (size_t)area)))
             return rc;
         HIP_TRY(spike::launch_spike_build((int)sp.size(), d_sp, d_kcols, d_ia, d_ja, d_a, d_W, nullptr));
         HIP_TRY(hipDeviceSynchronize());
@@ -322,9 +365,11 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         }
         HIP_TRY(hipDeviceSynchronize());
         dfree(f, d_W, sizeof(double) * std::max<int64_t>(area, 1));
-        dfree(f, d_a, sizeof(double) * pl.a.size());
-        dfree(f, d_ja, sizeof(int) * pl.ja.size());
-        dfree(f, d_ia, sizeof(int) * pl.ia.size());
+        if (!f->fp32) {
+            dfree(f, d_a, sizeof(double) * pl.a.size());
+            dfree(f, d_ja, sizeof(int) * pl.ja.size());
+            dfree(f, d_ia, sizeof(int) * pl.ia.size());
+        }
         dfree(f, d_wrow, sizeof(int64_t) * nred);
         dfree(f, d_cnt, sizeof(int) * nred);
         dfree(f, d_drop, sizeof(double) * nred);
@@ -467,6 +512,11 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.reduced_sweeps = f->sweeps;
     st.jacobi_bound = f->sweeps > 0 ? jbound : 0.0;
     st.rhs_per_launch = f->max_nr;
+    st.precision = f->opts.precision;
+    if (f->fp32 && (rc = calibrate_refinement(f))) return rc;
+    st.refine_steps = f->refine_steps;
+    st.refine_rate = f->refine_rate;
+    st.refine_est_err = f->refine_est;
     st.device_bytes = f->device_bytes;
     st.preproc_seconds = now_s() - t0;
     g_timers[0] += st.preproc_seconds;
@@ -485,11 +535,12 @@ bool is_device_ptr(const void *p)
     return attr.type == hipMemoryTypeDevice || attr.isManaged;
 }
 
-// nr right-hand sides (<= f->max_nr) in one pass through the kernels, all
-// pointers on the device.  ev (optional, 5 events) brackets the phases for
-// the profiling entry points.
-int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s,
-                hipEvent_t *ev)
+// One Spike solve of nr right-hand sides (<= f->max_nr), all pointers on the
+// device: x = M^-1 b, or x += M^-1 b with accum (refinement update).  ev
+// (optional, 5 events) brackets the phases; their device times are added to
+// ph[0..3].
+int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
+               hipEvent_t *ev, double *ph)
 {
     const Plan &pl = f->plan;
     int slots = 1;
@@ -519,6 +570,9 @@ int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx,
     ba.nr_slots = slots;
     ba.ldb = ldb;
     ba.ldx = ldx;
+    ba.fp32 = f->fp32 ? 1 : 0;
+    ba.cpl_val32 = f->d_cpl_val32;
+    ba.accum = accum;
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));
@@ -551,7 +605,104 @@ int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx,
         HIP_TRY(hipEventRecord(ev[2], s));
         HIP_TRY(hipEventRecord(ev[3], s));
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[4], s));
+    if (ev) {
+        HIP_TRY(hipEventRecord(ev[4], s));
+        HIP_TRY(hipEventSynchronize(ev[4]));
+        for (int i = 0; i < 4; i++) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            ph[i] += ms;
+        }
+    }
+    return SPIKE_OK;
+}
+
+// Solve of nr right-hand sides with the factor's refinement: x_0 = M^-1 b,
+// then refine_steps times r = b - A x (FP64), x += M^-1 r.  The residual's
+// device time is added to ph[2].
+int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s,
+                hipEvent_t *ev, double *ph)
+{
+    int rc = solve_pass(f, b, ldb, x, ldx, nr, 0, s, ev, ph);
+    if (rc) return rc;
+    const int n = f->plan.n;
+    for (int it = 0; it < f->refine_steps; it++) {
+        if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+        HIP_TRY(spike::launch_residual(n, f->d_ia, f->d_ja, f->d_a, b, ldb, x, ldx, f->d_r, n, nr,
+                                       f->plan.flip ? 1 : 0, s));
+        if (ev) {
+            HIP_TRY(hipEventRecord(ev[1], s));
+            HIP_TRY(hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            ph[2] += ms;
+        }
+        if ((rc = solve_pass(f, f->d_r, n, x, ldx, nr, 1, s, ev, ph))) return rc;
+    }
+    return SPIKE_OK;
+}
+
+// Refinement steps of an FP32 factor.  A random right-hand side (entries
+// uniform in (-1, 1), SplitMix64) is solved with refinement steps added one
+// at a time; c_j = max|x_j - x_{j-1}| / max|x_j| is the size of the j-th
+// correction, which estimates the relative error left after step j-1.  The
+// step count S is the smallest with c_{S+1} <= refine_tol.
+int calibrate_refinement(spike_factor_s *f)
+{
+    const int n = f->plan.n;
+    if (f->opts.refine_steps >= 0) {
+        f->refine_steps = f->opts.refine_steps;
+        return SPIKE_OK;
+    }
+    const double tol = f->opts.refine_tol > 0 ? f->opts.refine_tol : 1e-13;
+    constexpr int kMax = 8;
+    std::vector<double> hb(n), hx(n), hprev(n);
+    uint64_t st = 0x5350494B45434131ull;
+    for (int i = 0; i < n; i++) {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        hb[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+    }
+    double *db = nullptr, *dx = nullptr;
+    int rc;
+    if ((rc = f->dalloc(&db, (size_t)n)) || (rc = f->dalloc(&dx, (size_t)n))) return rc;
+    HIP_TRY(hipMemcpy(db, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    f->refine_steps = 0;
+    if ((rc = solve_pass(f, db, n, dx, n, 1, 0, nullptr, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpy(hprev.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<double> c;
+    int S = -1;
+    for (int j = 1; j <= kMax && S < 0; j++) {
+        HIP_TRY(spike::launch_residual(n, f->d_ia, f->d_ja, f->d_a, db, n, dx, n, f->d_r, n, 1,
+                                       f->plan.flip ? 1 : 0, nullptr));
+        if ((rc = solve_pass(f, f->d_r, n, dx, n, 1, 1, nullptr, nullptr, nullptr))) return rc;
+        HIP_TRY(hipMemcpy(hx.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+        double dm = 0.0, xm = 0.0;
+        for (int i = 0; i < n; i++) {
+            dm = std::max(dm, std::fabs(hx[i] - hprev[i]));
+            xm = std::max(xm, std::fabs(hx[i]));
+        }
+        c.push_back(xm > 0 ? dm / xm : 0.0);
+        if (!std::isfinite(c.back())) break;
+        if (c.back() <= tol) S = j - 1;
+        hprev.swap(hx);
+    }
+    dfree(f, db, sizeof(double) * n);
+    dfree(f, dx, sizeof(double) * n);
+    if (S < 0) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "FP32 refinement does not reach %.1e in %d steps (last correction %.2e)", tol,
+                      kMax, c.empty() ? 0.0 : c.back());
+        return fail(SPIKE_ERR_UNSUPPORTED, msg);
+    }
+    f->refine_steps = S;
+    f->refine_est = c[S];
+    double rate = 0.0;
+    for (int j = 1; j <= S; j++)
+        if (c[j - 1] > 0) rate = std::max(rate, c[j] / c[j - 1]);
+    f->refine_rate = S >= 1 ? rate : c[0];
     return SPIKE_OK;
 }
 
@@ -596,13 +747,11 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
         for (int i = 0; i < 5; i++) HIP_TRY(hipEventCreate(&ev[i]));
     for (int j = 0; j < nrhs; j += f->max_nr) {
         const int nr = std::min(f->max_nr, nrhs - j);
+        double ms[4] = {0.0, 0.0, 0.0, 0.0};
         int rc = solve_chunk(f, db + (size_t)j * ldb_d, ldb_d, dx + (size_t)j * ldx_d, ldx_d, nr, s,
-                             timed ? ev : nullptr);
+                             timed ? ev : nullptr, ms);
         if (rc) return rc;
         if (timed) {
-            HIP_TRY(hipEventSynchronize(ev[4]));
-            float ms[4];
-            for (int i = 0; i < 4; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
             if (phase_ms)
                 for (int i = 0; i < 4; i++) phase_ms[i] += ms[i];
             g_timers[5] += ms[0] * 1e-3;
@@ -694,6 +843,8 @@ void spike_options_default(spike_options_t *o)
     o->lanes_per_row = 0;
     o->verbose = 0;
     o->reduced_solver = SPIKE_REDUCED_AUTO;
+    o->refine_steps = -1;
+    o->refine_tol = 0.0;
 }
 
 int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, const int *ia, const int *ja,
@@ -707,7 +858,8 @@ int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, 
     spike_options_t o;
     spike_options_default(&o);
     if (opts) o = *opts;
-    if (o.precision != SPIKE_PREC_FP64) return fail(SPIKE_ERR_UNSUPPORTED, "unsupported precision mode");
+    if (o.precision != SPIKE_PREC_FP64 && o.precision != SPIKE_PREC_FP32_REFINED)
+        return fail(SPIKE_ERR_ARG, "unknown precision mode");
     spike_factor_s *f = new spike_factor_s();
     f->opts = o;
     int rc = factor_build(f, uplo, n, a, ia, ja);

@@ -1,4 +1,5 @@
-// gfx950 kernels of the Spike sparse triangular solve (FP64).
+// gfx950 kernels of the Spike sparse triangular solve (FP64, and FP32 block
+// solves for the iteratively refined mode).
 //
 // Block solve (pass 1 / pass 2): one 64-lane wavefront per partition.  The
 // partition's x segment lives in LDS (k+1 doubles; slot k holds 0.0 and is the
@@ -67,15 +68,56 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// Sum over aligned groups of G lanes; every lane of a group gets the same bits.
-template <int G>
-__device__ __forceinline__ double group_sum(double v)
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
 {
-    if constexpr (G >= 2) v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-    if constexpr (G >= 4) v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-    if constexpr (G >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror
-    if constexpr (G >= 16) v += dpp_f64<0x140>(v); // row_mirror
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) { return dpp_f64<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) { return dpp_f32<CTRL>(v); }
+
+// Sum over aligned groups of G lanes; every lane of a group gets the same bits.
+template <int G, class T>
+__device__ __forceinline__ T group_sum(T v)
+{
+    if constexpr (G >= 2) v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+    if constexpr (G >= 4) v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) v += dpp_move<0x141>(v);  // row_half_mirror
+    if constexpr (G >= 16) v += dpp_move<0x140>(v); // row_mirror
     return v;
+}
+
+// Value-type helpers of the block solve: FP64 or FP32 step values.
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <class T> struct VT;
+template <> struct VT<double> {
+    static __device__ __forceinline__ const double *val(const BlockArgs &A) { return A.st.val; }
+    static __device__ __forceinline__ const double *mult(const BlockArgs &A) { return A.st.mult; }
+    static __device__ __forceinline__ const double *cpl(const BlockArgs &A) { return A.cpl_val; }
+    static __device__ __forceinline__ double load(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s)
+    {
+        return buf_f64(r, v, s);
+    }
+};
+template <> struct VT<float> {
+    static __device__ __forceinline__ const float *val(const BlockArgs &A) { return A.st.val32; }
+    static __device__ __forceinline__ const float *mult(const BlockArgs &A) { return A.st.mult32; }
+    static __device__ __forceinline__ const float *cpl(const BlockArgs &A) { return A.cpl_val32; }
+    static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s)
+    {
+        return buf_f32(r, v, s);
+    }
+};
+
+// Writes one solution entry: x = v, or x += v for a refinement update.
+__device__ __forceinline__ void put_x(double *x, size_t i, double v, int accum)
+{
+    x[i] = accum ? x[i] + v : v;
 }
 
 // Pass-2 right-hand side of one partition in LDS, NR right-hand sides
@@ -84,63 +126,64 @@ __device__ __forceinline__ double group_sum(double v)
 // gathered lane-parallel into the LDS scratch prod[] and each coupling row
 // then sums its own products in entry order, so the result does not depend
 // on the lane count.  Right-hand sides r >= nr reuse column nr-1.
-template <int NR>
-__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, double *xs, double *prod,
-                                          bool scaled)
+template <int NR, class T>
+__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, T *prod, bool scaled)
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k, n = A.n, flip = A.flip;
     for (int i = lane; i < k; i += 64) {
         const int g = s + i, gm = flip ? n - 1 - g : g;
-        const double sc = scaled ? A.inv_diag[g] : 1.0;
+        const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int rr = r < A.nr ? r : A.nr - 1;
-            xs[i * NR + r] = A.b[(size_t)rr * A.ldb + gm] * sc;
+            xs[i * NR + r] = (T)A.b[(size_t)rr * A.ldb + gm] * sc;
         }
     }
     if (pt.ccnt == 0) return;
     if (A.cpl_fused) {
+        const T *cv = VT<T>::cpl(A);
         const int e0 = A.cpl_ptr[pt.cbeg], e1 = A.cpl_ptr[pt.cbeg + pt.ccnt];
 #pragma unroll 4
         for (int e = e0 + lane; e < e1; e += 64) {
-            const double w = A.cpl_val[e];
+            const T w = cv[e];
             const double *xc = A.xr + (size_t)A.cpl_col[e] * NR;
 #pragma unroll
-            for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * xc[r];
+            for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * (T)xc[r];
         }
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
             const int g = A.cpl_rows[pt.cbeg + c], gm = flip ? n - 1 - g : g;
-            const double sc = scaled ? A.inv_diag[g] : 1.0;
-            double acc[NR];
+            const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
+            T acc[NR];
 #pragma unroll
-            for (int r = 0; r < NR; r++) acc[r] = 0.0;
+            for (int r = 0; r < NR; r++) acc[r] = (T)0;
             for (int e = A.cpl_ptr[pt.cbeg + c]; e < A.cpl_ptr[pt.cbeg + c + 1]; e++)
 #pragma unroll
                 for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const int rr = r < A.nr ? r : A.nr - 1;
-                xs[(g - s) * NR + r] = (A.b[(size_t)rr * A.ldb + gm] - acc[r]) * sc;
+                xs[(g - s) * NR + r] = ((T)A.b[(size_t)rr * A.ldb + gm] - acc[r]) * sc;
             }
         }
     } else {
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
             const int g = A.cpl_rows[pt.cbeg + c];
-            const double sc = scaled ? A.inv_diag[g] : 1.0;
+            const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
 #pragma unroll
-            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = A.bc[(size_t)(pt.cbeg + c) * NR + r] * sc;
+            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = (T)A.bc[(size_t)(pt.cbeg + c) * NR + r] * sc;
         }
     }
 }
 
 // Block solve with NR right-hand sides sharing each step record; x segments
 // interleaved in LDS as xs[i*NR + r] (i < k+2, slot k holds 0).
-template <int G, int M, int D, int PASS, int NR>
+template <int G, int M, int D, int PASS, int NR, class T>
 __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 {
-    extern __shared__ double xs[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_blk[];
+    T *xs = reinterpret_cast<T *>(smem_blk);
     constexpr int R = 64 / G;
     const int p = blockIdx.x;
     const DevPart pt = A.parts[p];
@@ -160,24 +203,25 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const int rr = r < A.nr ? r : A.nr - 1;
-                xs[i * NR + r] = A.b[(size_t)rr * A.ldb + gm];
+                xs[i * NR + r] = (T)A.b[(size_t)rr * A.ldb + gm];
             }
         }
     } else {
-        pass2_rhs<NR>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false);
+        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false);
     }
     if (lane == 0)
 #pragma unroll
-        for (int r = 0; r < NR; r++) xs[k * NR + r] = 0.0;
+        for (int r = 0; r < NR; r++) xs[k * NR + r] = (T)0;
     __syncthreads();
 
+    constexpr unsigned TB = sizeof(T);
     const int64_t sb = A.st.part_begin[p];
     const int nst = (int)(A.st.part_begin[p + 1] - sb);
     const unsigned nstu = (unsigned)(nst > 0 ? nst : 1);
-    const __amdgpu_buffer_rsrc_t bval = make_rsrc(A.st.val + (size_t)sb * M * 64, nstu * M * 512u);
+    const __amdgpu_buffer_rsrc_t bval = make_rsrc(VT<T>::val(A) + (size_t)sb * M * 64, nstu * M * 64u * TB);
     const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
     const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
-    const __amdgpu_buffer_rsrc_t bmul = make_rsrc(A.st.mult + (size_t)sb * R, nstu * R * 8u);
+    const __amdgpu_buffer_rsrc_t bmul = make_rsrc(VT<T>::mult(A) + (size_t)sb * R, nstu * R * TB);
     // Register ring of D step records.  It starts as empty padding steps
     // (column k, no row) and the loop starts D steps early, so the first D
     // iterations compute nothing and only issue loads.  Every load then comes
@@ -187,20 +231,20 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     // loads stay D steps in flight.
     // Columns and rows stay 16-bit in the ring: widening them right after the
     // load would make the compiler wait for the load at the loop back-edge.
-    double rv[D][M];
+    T rv[D][M];
     uint16_t rc[D][M];
     uint16_t rrow[D];
-    double rmul[D];
+    T rmul[D];
     if (nst <= 0) goto epilogue;
 #pragma unroll
     for (int d = 0; d < D; d++) {
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            rv[d][m] = 0.0;
+            rv[d][m] = (T)0;
             rc[d][m] = (uint16_t)k;
         }
         rrow[d] = (uint16_t)0xFFFFu;
-        rmul[d] = 0.0;
+        rmul[d] = (T)0;
     }
     {
         // One step from ring slot d: LDS reads first (operands and the row's
@@ -211,7 +255,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             const unsigned row = rrow[d];
             const bool valid = row != 0xFFFFu;
             const unsigned arow = valid ? row : (unsigned)k;
-            double xv[M][NR], bv[NR], t[NR];
+            T xv[M][NR], bv[NR], t[NR];
 #pragma unroll
             for (int m = 0; m < M; m++)
 #pragma unroll
@@ -220,7 +264,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             for (int r = 0; r < NR; r++) bv[r] = xs[arow * NR + r];
 #pragma unroll
             for (int r = 0; r < NR; r++) {
-                t[r] = 0.0;
+                t[r] = (T)0;
 #pragma unroll
                 for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
                 t[r] = li == 0 ? t[r] + bv[r] : t[r];
@@ -236,11 +280,11 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             const unsigned nxt = (unsigned)(sidx + D < nst ? sidx + D : nst - 1);
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                rv[d][m] = buf_f64(bval, (unsigned)(m * 64 + lane) * 8u, nxt * (unsigned)(M * 512));
+                rv[d][m] = VT<T>::load(bval, (unsigned)(m * 64 + lane) * TB, nxt * (unsigned)(M * 64) * TB);
                 rc[d][m] = buf_u16(bcol, (unsigned)(m * 64 + lane) * 2u, nxt * (unsigned)(M * 128));
             }
             rrow[d] = buf_u16(brow, (unsigned)slot * 2u, nxt * (unsigned)(R * 2));
-            rmul[d] = buf_f64(bmul, (unsigned)slot * 8u, nxt * (unsigned)(R * 8));
+            rmul[d] = VT<T>::load(bmul, (unsigned)slot * TB, nxt * (unsigned)R * TB);
         };
         int base = -D;
         for (; base + D <= nst; base += D) {
@@ -261,7 +305,7 @@ epilogue:
         for (int q = lane; q < pt.dcnt; q += 64) {
             const int i = A.dep_rows[pt.dbeg + q] - s;
 #pragma unroll
-            for (int r = 0; r < NR; r++) A.xred[(size_t)(pt.dbeg + q) * NR + r] = xs[i * NR + r];
+            for (int r = 0; r < NR; r++) A.xred[(size_t)(pt.dbeg + q) * NR + r] = (double)xs[i * NR + r];
         }
     // pass 2 writes the solution; without coupling, pass 1's result is final
     if (PASS == 2 || pt.f >= k)
@@ -269,7 +313,7 @@ epilogue:
             const int g = s + i, gm = flip ? n - 1 - g : g;
 #pragma unroll
             for (int r = 0; r < NR; r++)
-                if (r < A.nr) A.x[(size_t)r * A.ldx + gm] = xs[i * NR + r];
+                if (r < A.nr) put_x(A.x, (size_t)r * A.ldx + gm, (double)xs[i * NR + r], A.accum);
         }
 }
 
@@ -302,7 +346,7 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
             xs[r] = A.b[flip ? n - 1 - g : g] * invd[g];
         }
     } else {
-        pass2_rhs<1>(A, pt, xs, xs + A.kmax + 2, true);
+        pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + 2, true);
     }
     if (lane == 0) {
         xs[k] = 0.0;
@@ -382,22 +426,28 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
     if (PASS == 2 || pt.f >= k)
         for (int r = lane; r < k; r += 64) {
             const int g = s + r;
-            A.x[flip ? n - 1 - g : g] = xs[r];
+            put_x(A.x, flip ? n - 1 - g : g, xs[r], A.accum);
         }
 }
 
-template <int G, int D, int PASS, int NR>
-hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
+template <int G, int D, int PASS, int NR, class T>
+hipError_t launch_gdt(int P, const BlockArgs &a, hipStream_t s)
 {
     constexpr int M = 1;
-    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * NR * sizeof(double);
-    auto kern = k_block_solve<G, M, D, PASS, NR>;
+    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * NR * sizeof(T);
+    auto kern = k_block_solve<G, M, D, PASS, NR, T>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(P), dim3(64), lds, s, a);
     return hipGetLastError();
+}
+
+template <int G, int D, int PASS, int NR>
+hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
+{
+    return a.fp32 ? launch_gdt<G, D, PASS, NR, float>(P, a, s) : launch_gdt<G, D, PASS, NR, double>(P, a, s);
 }
 
 template <int G, int PASS>
@@ -429,7 +479,7 @@ template <int PASS>
 hipError_t launch_pass(int P, const BlockArgs &a, hipStream_t s)
 {
     if (a.st.fwd) {
-        if (a.st.G != 16 || a.nr_slots != 1) return hipErrorInvalidValue;
+        if (a.st.G != 16 || a.nr_slots != 1 || a.fp32) return hipErrorInvalidValue;
         return a.ring >= 16 ? launch_fwd<16, PASS>(P, a, s) : launch_fwd<8, PASS>(P, a, s);
     }
     switch (a.st.G) {
@@ -567,6 +617,43 @@ __global__ __launch_bounds__(256) void k_coupling(int ncpl, const int *__restric
     for (int t = 0; t < NR; t++) {
         const int tt = t < nr ? t : nr - 1;
         bc[(size_t)c * NR + t] = b[(size_t)tt * ldb + gm] - acc[t];
+    }
+}
+
+// ---------------------------------------------------------------- residual
+
+// r = b - A x in FP64, G = 8 lanes per lower-form row (strided over its
+// entries), NR right-hand sides per pass (nr active).  Lane sums are combined
+// with DPP in a fixed order.
+template <int NR>
+__global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                  const double *__restrict__ a, const double *__restrict__ b,
+                                                  int ldb, const double *__restrict__ x, int ldx,
+                                                  double *__restrict__ r, int ldr, int nr, int flip)
+{
+    constexpr int G = 8;
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int row = gid / G, li = gid % G;
+    double acc[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = 0.0;
+    if (row < n) {
+        const int e1 = ia[row + 1];
+        for (int e = ia[row] + li; e < e1; e += G) {
+            const double v = a[e];
+            const int c = ja[e], cm = flip ? n - 1 - c : c;
+#pragma unroll
+            for (int t = 0; t < NR; t++)
+                if (t < nr) acc[t] = fma(v, x[(size_t)t * ldx + cm], acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
+    if (row < n && li == 0) {
+        const int gm = flip ? n - 1 - row : row;
+#pragma unroll
+        for (int t = 0; t < NR; t++)
+            if (t < nr) r[(size_t)t * ldr + gm] = b[(size_t)t * ldb + gm] - acc[t];
     }
 }
 
@@ -750,6 +837,26 @@ hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, co
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_residual(int n, const int *ia, const int *ja, const double *a, const double *b, int ldb,
+                           const double *x, int ldx, double *r, int ldr, int nr, int flip, hipStream_t s)
+{
+    if (n <= 0 || nr <= 0) return hipSuccess;
+    const dim3 grid((unsigned)(((size_t)n * 8 + 255) / 256)), block(256);
+    for (int j = 0; j < nr; j += 4) {
+        const int c = nr - j < 4 ? nr - j : 4;
+        const double *bj = b + (size_t)j * ldb;
+        const double *xj = x + (size_t)j * ldx;
+        double *rj = r + (size_t)j * ldr;
+        if (c == 1)
+            hipLaunchKernelGGL(k_residual<1>, grid, block, 0, s, n, ia, ja, a, bj, ldb, xj, ldx, rj, ldr, c, flip);
+        else
+            hipLaunchKernelGGL(k_residual<4>, grid, block, 0, s, n, ia, ja, a, bj, ldb, xj, ldx, rj, ldr, c, flip);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_spike_build(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,

@@ -22,6 +22,8 @@ struct DevSteps {
     const uint16_t *row;
     const double *mult;          // not fwd
     const double *late;          // fwd: R x R block per step
+    const float *val32;          // FP32 factors: values and multipliers
+    const float *mult32;
     const int64_t *part_begin;   // P+1
     int G, M, R;
     int fwd;
@@ -54,6 +56,12 @@ struct BlockArgs {
     // nr_slots interleaved values per index
     int nr, nr_slots;
     int ldb, ldx;
+    // FP32 factors: step values, multipliers and coupling values are FP32 and
+    // the block solve accumulates in FP32 (b, xred, xr and x stay FP64)
+    int fp32;
+    const float *cpl_val32;
+    // 1: the solve's result is added to x (refinement update x += d)
+    int accum;
 };
 
 // Pass 1 (y = D^-1 b, written at the dependents; also the solution of
@@ -98,6 +106,12 @@ hipError_t launch_reduced_count(int nred, const int64_t *wrow, const int *kcnt, 
 hipError_t launch_reduced_fill(int nred, const int64_t *wrow, const int *kbeg, const int *kcnt,
                                const double *tol, const double *W, const int *kridx,
                                const int *rp, int *ci, double *v, hipStream_t s);
+
+// r = b - A x in FP64 over the lower-form CSR (ia, ja, a) for nr right-hand
+// sides (columns of b, x, r with leading dimensions ldb, ldx, ldr); with flip,
+// the caller's index of lower-form row i is n-1-i.
+hipError_t launch_residual(int n, const int *ia, const int *ja, const double *a, const double *b, int ldb,
+                           const double *x, int ldx, double *r, int ldr, int nr, int flip, hipStream_t s);
 
 // Largest dynamic LDS a block-solve workgroup may use, in bytes.
 size_t block_solve_lds_limit();

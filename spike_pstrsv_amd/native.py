@@ -33,7 +33,8 @@ EXPORTED = [
 
 class Options(C.Structure):
     _fields_ = [("partitions", C.c_int), ("drop_tol", C.c_double), ("precision", C.c_int),
-                ("lanes_per_row", C.c_int), ("verbose", C.c_int), ("reduced_solver", C.c_int)]
+                ("lanes_per_row", C.c_int), ("verbose", C.c_int), ("reduced_solver", C.c_int),
+                ("refine_steps", C.c_int), ("refine_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -42,7 +43,8 @@ class Stats(C.Structure):
                 ("dropped_rowsum_max", C.c_double), ("steps_pass1", C.c_int), ("steps_pass2", C.c_int),
                 ("lanes_per_row", C.c_int), ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double),
                 ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double),
-                ("rhs_per_launch", C.c_int)]
+                ("rhs_per_launch", C.c_int), ("precision", C.c_int), ("refine_steps", C.c_int),
+                ("refine_rate", C.c_double), ("refine_est_err", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -121,10 +123,12 @@ def _csr(a, ia, ja):
 
 
 REDUCED_SOLVERS = {"auto": 0, "exact": 1, "jacobi": 2}
+PRECISIONS = {"fp64": 0, "fp32_refined": 1}
 
 
 def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False,
-            reduced_solver: str = "auto") -> Options:
+            reduced_solver: str = "auto", precision: str = "fp64", refine_steps: int = -1,
+            refine_tol: float = 0.0) -> Options:
     o = Options()
     lib.spike_options_default(C.byref(o))
     o.partitions = partitions
@@ -132,6 +136,9 @@ def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0,
     o.lanes_per_row = lanes_per_row
     o.verbose = int(verbose)
     o.reduced_solver = REDUCED_SOLVERS[reduced_solver]
+    o.precision = PRECISIONS[precision]
+    o.refine_steps = refine_steps
+    o.refine_tol = refine_tol
     return o
 
 
@@ -139,11 +146,13 @@ class Factor:
     """A preprocessed matrix resident in device memory (spike_factor_t)."""
 
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
-                 lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto"):
+                 lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto",
+                 precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()
-        o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver)
+        o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver, precision, refine_steps,
+                    refine_tol)
         _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
                                        ja.ctypes.data, C.byref(o)))
         self.handle = h.value

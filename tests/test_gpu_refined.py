@@ -1,0 +1,106 @@
+"""GPU parity of the FP32-valued solve with FP64 iterative refinement.
+
+Bar: relative error max|x - x_ref| / max|x_ref| <= 1e-12 against sequential
+FP64 substitution (the oracle's ldsol / udsol), the same bar as the FP64
+path.  The unrefined FP32 solve is checked to be an FP32-accurate result
+(error well above FP64 rounding), so a test cannot pass on an FP64 path that
+ignores the precision option.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+from spike_pstrsv_amd import gen
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def relerr(x, ref):
+    return float(np.max(np.abs(x - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def native():
+    import torch
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    from spike_pstrsv_amd import native as nat
+
+    return nat
+
+
+def test_refined_lap2d_256(native):
+    m = gen.lap2d(256)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 101)
+    ref = orc.ldsol(m.a, m.ia, m.ja, b)
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=8, precision="fp32_refined")
+    st = f.stats()
+    print("lap2d_256 fp32_refined", st)
+    assert st["precision"] == 1 and st["refine_steps"] >= 1
+    assert st["refine_est_err"] <= 1e-13
+    assert relerr(f.solve(b), ref) <= TOL
+
+
+def test_unrefined_fp32_is_fp32_accurate(native):
+    m = gen.banded(1 << 16, seed=gen.SEED_BASE + 102)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 103)
+    ref = orc.ldsol(m.a, m.ia, m.ja, b)
+    f0 = native.Factor("L", m.a, m.ia, m.ja, partitions=64, precision="fp32_refined", refine_steps=0)
+    e0 = relerr(f0.solve(b), ref)
+    print("fp32 without refinement: relerr", e0)
+    assert 1e-10 < e0 < 1e-5
+    f1 = native.Factor("L", m.a, m.ia, m.ja, partitions=64, precision="fp32_refined", refine_steps=1)
+    e1 = relerr(f1.solve(b), ref)
+    print("fp32 + 1 refinement step: relerr", e1)
+    assert e1 < e0 * 1e-4
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("nrhs", [1, 5, 8])
+def test_refined_banded_multi_rhs(native, uplo, nrhs):
+    m = gen.banded(1 << 16, seed=gen.SEED_BASE + 110 + nrhs)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    B = gen.rhs(m.n, nrhs, seed=gen.SEED_BASE + 111)
+    f = native.Factor(uplo, m.a, m.ia, m.ja, partitions=64, precision="fp32_refined")
+    X = f.solve(B, nrhs=nrhs).reshape(nrhs, m.n)
+    for j in range(nrhs):
+        assert relerr(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B.reshape(nrhs, m.n)[j])) <= TOL
+
+
+def test_refined_lap3d_32_multi_rhs(native):
+    """3D 7-point Laplacian lower triangle (diag 6, off -1): plane coupling."""
+    m = gen.lap3d(32)
+    B = gen.rhs(m.n, 4, seed=gen.SEED_BASE + 120)
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=16, precision="fp32_refined")
+    print("lap3d_32 fp32_refined", f.stats())
+    X = f.solve(B, nrhs=4)
+    for j in range(4):
+        assert relerr(X[j], orc.ldsol(m.a, m.ia, m.ja, B[j])) <= TOL
+
+
+def test_refined_matches_fp64_path(native):
+    m = gen.banded(1 << 15, seed=gen.SEED_BASE + 130)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 131)
+    x64 = native.Factor("L", m.a, m.ia, m.ja, partitions=32).solve(b)
+    x32 = native.Factor("L", m.a, m.ia, m.ja, partitions=32, precision="fp32_refined").solve(b)
+    assert relerr(x32, x64) <= TOL
+
+
+def test_refined_device_pointers_and_profiled(native):
+    import torch
+
+    m = gen.banded(1 << 15, seed=gen.SEED_BASE + 140)
+    B = gen.rhs(m.n, 3, seed=gen.SEED_BASE + 141)
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=32, precision="fp32_refined")
+    X = f.solve(B, nrhs=3)
+    db = torch.from_numpy(B).cuda()
+    dx = torch.zeros_like(db)
+    s = torch.cuda.current_stream().cuda_stream
+    f.solve(db.data_ptr(), dx.data_ptr(), nrhs=3, stream=s)
+    torch.cuda.synchronize()
+    assert np.array_equal(dx.cpu().numpy(), X)
+    ph = f.solve_profiled(db.data_ptr(), dx.data_ptr(), nrhs=3, stream=s)
+    assert len(ph) == 4 and all(v > 0 for v in ph)
+    assert np.array_equal(dx.cpu().numpy(), X)
