@@ -167,6 +167,74 @@ int spike_factor_stats(spike_factor_t f, spike_stats_t *st);
  * (src/spike_pstrsv_ops.c:1311-1333), written to stdout. */
 int spike_factor_log(spike_factor_t f);
 
+/* ------------------------------------------------------------------ */
+/* Batches: many independent matrices solved together                  */
+/* ------------------------------------------------------------------ */
+
+/* One matrix of a batch (host CSR, same conventions as above). */
+typedef struct {
+    int n;
+    const double *a;
+    const int *ia;
+    const int *ja;
+    int nrhs;                 /* right-hand sides of this matrix, 1..8         */
+} spike_batch_matrix_t;
+
+typedef struct {
+    int rows_per_partition;   /* Spike partition size; 0 = autotuned           */
+    int lanes_per_row;        /* 0 = per matrix from its row lengths           */
+    int precision;            /* SPIKE_PREC_*                                  */
+    int use_graph;            /* 1 = solves replay a captured hipGraph         */
+    int autotune_sample;      /* most matrices the autotuner times per RHS-slot */
+                              /* class; 0 = enough to fill the GPU             */
+    double drop_tol;          /* as in spike_options_t                         */
+    double refine_tol;        /* as in spike_options_t                         */
+} spike_batch_options_t;
+
+typedef struct spike_batch_s *spike_batch_t;
+
+typedef struct {
+    int count;                /* matrices                                      */
+    int groups;               /* sub-factors (RHS slots x lanes per row)       */
+    int64_t rows;             /* sum of n                                      */
+    int64_t nnz;
+    int64_t rows_rhs;         /* sum of n * nrhs                               */
+    int64_t partitions;
+    int64_t n_reduced;
+    int rows_per_partition[4];/* in use at 1, 2, 4, 8 RHS slots (autotuned    */
+                              /* per slot class when 0 was given)              */
+    int refine_steps;
+    double refine_est_err;
+    int64_t device_bytes;
+    double preproc_seconds;
+    double autotune_seconds;
+    int graph_nodes;          /* kernel launches per solve                     */
+} spike_batch_stats_t;
+
+void spike_batch_options_default(spike_batch_options_t *o);
+
+/* Preprocesses count matrices (all with the same uplo) into one batch held in
+ * device memory.  Matrices are grouped by right-hand-side slots (1, 2, 4, 8)
+ * and lanes per row; each group is one block-diagonal Spike factor whose
+ * partitions never straddle two matrices. */
+int spike_batch_create(spike_batch_t *bt, char uplo, int count, const spike_batch_matrix_t *m,
+                       const spike_batch_options_t *opts);
+
+/* Packed layout of a batch's right-hand sides and solutions: matrix i's nrhs
+ * columns start at element spike_batch_offset(bt, i), column-major with
+ * leading dimension n_i; spike_batch_length(bt) elements in total. */
+int64_t spike_batch_offset(spike_batch_t bt, int i);
+int64_t spike_batch_length(spike_batch_t bt);
+
+/* Solves every matrix of the batch for its right-hand sides (packed layout;
+ * host or device pointers).  With use_graph, the launch sequence of every
+ * group is captured once per (b, x) pair into a hipGraph, the groups on
+ * parallel branches, and replayed on stream. */
+int spike_batch_solve(spike_batch_t bt, const double *b, double *x, void *stream);
+
+int spike_batch_stats(spike_batch_t bt, spike_batch_stats_t *st);
+int spike_batch_destroy(spike_batch_t bt);
+
 /* Host-only symbolic analysis (no device needed): fills st with the sizes the
  * factorisation would have, except fields that need the spikes (nnz_reduced,
  * reduced_levels, dropped_rowsum_max, device_bytes are 0). */

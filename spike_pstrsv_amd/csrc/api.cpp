@@ -89,6 +89,20 @@ struct spike_factor_s {
     double *d_r = nullptr;
     int refine_steps = 0;
     double refine_rate = 0.0, refine_est = 0.0;
+    // Matrices held by the factor.  Empty: one matrix, addressed by the
+    // solve call's leading dimensions and RHS count.  Otherwise a
+    // block-diagonal batch: matrix i owns lower-form rows [row0, row0+n) and
+    // its nrhs right-hand sides start at element boff of the packed b and x
+    // (column-major, leading dimension n); batch_flip: inputs were 'U'.
+    struct Mat {
+        int row0, n, nrhs;
+        int64_t boff;
+    };
+    std::vector<Mat> mats;
+    bool batch_flip = false;
+    int batch_slots = 0;        // RHS slots of every launch of a batch factor
+    int batch_max_nrhs = 0;
+    int64_t vec_len = 0;        // batch: packed length of b and x
 
     template <class T>
     int dalloc(T **p, size_t count)
@@ -209,6 +223,10 @@ void choose_rhs_slots(spike_factor_s *f)
         const int c = 1 << li;
         if (c > 1 && pl.steps.fwd) break;
         if (x_per * c > lim) break;
+        if (f->batch_slots > 0 && c != f->batch_slots) {
+            if (c > f->batch_slots) break;
+            continue;
+        }
         const long r1 = rounds(x_per * c);
         const bool fuse = f->cpl_fused && (x_per + c_per) * c <= lim && rounds((x_per + c_per) * c) == r1;
         f->fuse_at[li] = fuse;
@@ -223,7 +241,8 @@ void choose_rhs_slots(spike_factor_s *f)
 
 int calibrate_refinement(spike_factor_s *f);
 
-int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int *ia, const int *ja)
+int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int *ia, const int *ja,
+                 const std::vector<int> *starts = nullptr)
 {
     const double t0 = now_s();
     int rc;
@@ -235,6 +254,10 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     P = std::min(P, n);
     // raise P until the largest partition fits the LDS x segment
     while (P < n && n - (n / P) * (P - 1) > po.kmax_limit) P++;
+    if (starts) {
+        po.starts = *starts;
+        P = (int)starts->size() - 1;
+    }
     f->fp32 = f->opts.precision == SPIKE_PREC_FP32_REFINED;
     if (const char *e = std::getenv("SPIKE_PSTRSV_FORWARD")) po.forward = std::atoi(e) != 0 && !f->fp32;
     po.P = P;
@@ -248,9 +271,22 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 
     // ---- per-partition records, dependents, coupling rows
     std::vector<DevPart> hp(P);
-    for (int t = 0; t < P; t++) {
+    for (int t = 0, mi = 0; t < P; t++) {
         const spike::Part &pt = pl.parts[t];
-        hp[t] = DevPart{pt.s, pt.k, pt.f, pt.hi1, pt.dbeg, pt.dcnt, pt.cbeg, pt.ccnt};
+        hp[t] = DevPart{pt.s, pt.k, pt.f, pt.hi1, pt.dbeg, pt.dcnt, pt.cbeg, pt.ccnt,
+                        0, n, pl.flip ? 1 : 0, 0, 0, 0, 0};
+        if (!f->mats.empty()) {
+            while (mi + 1 < (int)f->mats.size() && f->mats[mi].row0 + f->mats[mi].n <= pt.s) mi++;
+            const spike_factor_s::Mat &m = f->mats[mi];
+            if (pt.s < m.row0 || pt.s + pt.k > m.row0 + m.n)
+                return fail(SPIKE_ERR_ARG, "a partition straddles two matrices of the batch");
+            hp[t].row0 = m.row0;
+            hp[t].mn = m.n;
+            hp[t].mflip = f->batch_flip ? 1 : 0;
+            hp[t].ld = m.n;
+            hp[t].mnr = m.nrhs;
+            hp[t].boff = m.boff;
+        }
     }
     if ((rc = f->upload(&f->d_parts, hp)) || (rc = f->upload(&f->d_dep_rows, pl.dep_rows)) ||
         (rc = f->upload(&f->d_cpl_rows, pl.cpl_rows)) || (rc = f->upload(&f->d_cpl_ptr, pl.cpl_ptr)) ||
@@ -266,11 +302,15 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         f->cmax = std::max(f->cmax, pl.cpl_ptr[pt.cbeg + pt.ccnt] - pl.cpl_ptr[pt.cbeg]);
     }
     choose_rhs_slots(f);
+    if (f->batch_slots > 0 && f->max_nr != f->batch_slots)
+        return fail(SPIKE_ERR_UNSUPPORTED, "partitions too large for " + std::to_string(f->batch_slots) +
+                                               " right-hand sides in LDS; use fewer rows per partition");
     if ((rc = f->dalloc(&f->d_y, (size_t)nred * f->max_nr)) || (rc = f->dalloc(&f->d_xa, (size_t)nred * f->max_nr)) ||
         (rc = f->dalloc(&f->d_xb, (size_t)nred * f->max_nr)) ||
         (rc = f->dalloc(&f->d_bc, pl.cpl_rows.size() * f->max_nr)))
         return rc;
-    if (f->fp32 && (rc = f->dalloc(&f->d_r, (size_t)n * f->max_nr))) return rc;
+    // (a batch shares one residual workspace across its groups)
+    if (f->fp32 && f->mats.empty() && (rc = f->dalloc(&f->d_r, (size_t)n * f->max_nr))) return rc;
 
     // lower-form CSR on the device: the spike build reads it, the FP32 mode
     // keeps it for the FP64 residual
@@ -458,9 +498,12 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
             }
         }
     }
-    // cost model (microseconds): level solve ~0.15 per level in LDS, ~1 in
-    // L2; a sweep ~2.5 launch + streaming the reduced system at 4 TB/s
-    const double t_exact = maxlev * (f->red_in_lds ? 0.15 : 1.0);
+    // cost model (microseconds): the level solve runs in one workgroup,
+    // ~0.15 per level in LDS or ~1 in L2, plus ~4 ns per entry and RHS slot
+    // (one workgroup streams the reduced system, measured); a sweep is ~2.5
+    // launch + the reduced system streamed by the whole GPU at ~4 TB/s
+    const double slots_avg = f->batch_slots > 0 ? f->batch_slots : 1.0;
+    const double t_exact = maxlev * (f->red_in_lds ? 0.15 : 1.0) + (double)rnnz * slots_avg * 4e-3;
     const double t_jacobi = sweeps < 0 ? 1e30 : sweeps * (2.5 + (double)rnnz * 12.0 / 4e6);
     int mode = f->opts.reduced_solver;
     if (mode == SPIKE_REDUCED_JACOBI && sweeps < 0)
@@ -535,17 +578,13 @@ bool is_device_ptr(const void *p)
     return attr.type == hipMemoryTypeDevice || attr.isManaged;
 }
 
-// One Spike solve of nr right-hand sides (<= f->max_nr), all pointers on the
-// device: x = M^-1 b, or x += M^-1 b with accum (refinement update).  ev
-// (optional, 5 events) brackets the phases; their device times are added to
-// ph[0..3].
-int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
-               hipEvent_t *ev, double *ph)
+// Launch arguments of one solve of nr right-hand sides (<= f->max_nr).
+spike::BlockArgs make_args(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum)
 {
     const Plan &pl = f->plan;
-    int slots = 1;
+    int slots = f->batch_slots > 0 ? f->batch_slots : 1;
     while (slots < nr) slots <<= 1;
-    spike::BlockArgs ba;
+    spike::BlockArgs ba{};
     ba.parts = f->d_parts;
     ba.st = f->steps;
     ba.b = b;
@@ -559,8 +598,7 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
     ba.kmax = pl.kmax;
     ba.ring = f->ring;
     ba.inv_diag = f->d_inv_diag;
-    const bool fuse = f->fuse_at[__builtin_ctz((unsigned)slots)];
-    ba.cpl_fused = fuse ? 1 : 0;
+    ba.cpl_fused = f->fuse_at[__builtin_ctz((unsigned)slots)] ? 1 : 0;
     ba.cmax = f->cmax;
     ba.cpl_ptr = f->d_cpl_ptr;
     ba.cpl_col = f->d_cpl_col;
@@ -573,6 +611,19 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
     ba.fp32 = f->fp32 ? 1 : 0;
     ba.cpl_val32 = f->d_cpl_val32;
     ba.accum = accum;
+    return ba;
+}
+
+// One Spike solve of nr right-hand sides (<= f->max_nr), all pointers on the
+// device: x = M^-1 b, or x += M^-1 b with accum (refinement update).  ev
+// (optional, 5 events) brackets the phases; their device times are added to
+// ph[0..3].
+int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
+               hipEvent_t *ev, double *ph)
+{
+    const Plan &pl = f->plan;
+    spike::BlockArgs ba = make_args(f, b, ldb, x, ldx, nr, accum);
+    const int slots = ba.nr_slots;
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));
@@ -596,9 +647,9 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
         }
         if (ev) HIP_TRY(hipEventRecord(ev[2], s));
         ba.xr = xr;
-        if (!fuse)
-            HIP_TRY(spike::launch_coupling((int)pl.cpl_rows.size(), f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
-                                           f->d_cpl_val, xr, b, ldb, nr, slots, f->d_bc, pl.n, ba.flip, s));
+        if (!ba.cpl_fused)
+            HIP_TRY(spike::launch_coupling(pl.P, f->d_parts, f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
+                                           f->d_cpl_val, xr, ba, f->d_bc, s));
         if (ev) HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
     } else if (ev) {
@@ -617,6 +668,15 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
     return SPIKE_OK;
 }
 
+// r = b - A x (FP64) into the factor's residual workspace.
+int residual(spike_factor_s *f, const double *b, int ldb, const double *x, int ldx, int nr, hipStream_t s)
+{
+    spike::BlockArgs ra = make_args(f, b, ldb, const_cast<double *>(x), ldx, nr, 0);
+    const int nr_max = f->mats.empty() ? nr : f->batch_max_nrhs;
+    HIP_TRY(spike::launch_residual(f->plan.P, f->d_parts, f->d_ia, f->d_ja, f->d_a, ra, nr_max, f->d_r, f->plan.n, s));
+    return SPIKE_OK;
+}
+
 // Solve of nr right-hand sides with the factor's refinement: x_0 = M^-1 b,
 // then refine_steps times r = b - A x (FP64), x += M^-1 r.  The residual's
 // device time is added to ph[2].
@@ -628,8 +688,7 @@ int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx,
     const int n = f->plan.n;
     for (int it = 0; it < f->refine_steps; it++) {
         if (ev) HIP_TRY(hipEventRecord(ev[0], s));
-        HIP_TRY(spike::launch_residual(n, f->d_ia, f->d_ja, f->d_a, b, ldb, x, ldx, f->d_r, n, nr,
-                                       f->plan.flip ? 1 : 0, s));
+        if ((rc = residual(f, b, ldb, x, ldx, nr, s))) return rc;
         if (ev) {
             HIP_TRY(hipEventRecord(ev[1], s));
             HIP_TRY(hipEventSynchronize(ev[1]));
@@ -643,10 +702,11 @@ int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx,
 }
 
 // Refinement steps of an FP32 factor.  A random right-hand side (entries
-// uniform in (-1, 1), SplitMix64) is solved with refinement steps added one
-// at a time; c_j = max|x_j - x_{j-1}| / max|x_j| is the size of the j-th
-// correction, which estimates the relative error left after step j-1.  The
-// step count S is the smallest with c_{S+1} <= refine_tol.
+// uniform in (-1, 1), SplitMix64; every column of every matrix of a batch) is
+// solved with refinement steps added one at a time; c_j = max over matrices
+// of max|x_j - x_{j-1}| / max|x_j| is the size of the j-th correction, which
+// estimates the relative error left after step j-1.  The step count S is the
+// smallest with c_{S+1} <= refine_tol.
 int calibrate_refinement(spike_factor_s *f)
 {
     const int n = f->plan.n;
@@ -656,9 +716,17 @@ int calibrate_refinement(spike_factor_s *f)
     }
     const double tol = f->opts.refine_tol > 0 ? f->opts.refine_tol : 1e-13;
     constexpr int kMax = 8;
-    std::vector<double> hb(n), hx(n), hprev(n);
+    const bool bat = !f->mats.empty();
+    const size_t len = bat ? (size_t)f->vec_len : (size_t)n;
+    const int nr = bat ? f->batch_slots : 1;
+    std::vector<std::pair<size_t, size_t>> seg;   // per matrix [begin, end) of its columns
+    if (bat)
+        for (const auto &m : f->mats) seg.emplace_back((size_t)m.boff, (size_t)m.boff + (size_t)m.n * m.nrhs);
+    else
+        seg.emplace_back(0, (size_t)n);
+    std::vector<double> hb(len), hx(len), hprev(len);
     uint64_t st = 0x5350494B45434131ull;
-    for (int i = 0; i < n; i++) {
+    for (size_t i = 0; i < len; i++) {
         uint64_t z = (st += 0x9E3779B97F4A7C15ull);
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -667,30 +735,34 @@ int calibrate_refinement(spike_factor_s *f)
     }
     double *db = nullptr, *dx = nullptr;
     int rc;
-    if ((rc = f->dalloc(&db, (size_t)n)) || (rc = f->dalloc(&dx, (size_t)n))) return rc;
-    HIP_TRY(hipMemcpy(db, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    if ((rc = f->dalloc(&db, len)) || (rc = f->dalloc(&dx, len))) return rc;
+    HIP_TRY(hipMemcpy(db, hb.data(), sizeof(double) * len, hipMemcpyHostToDevice));
     f->refine_steps = 0;
-    if ((rc = solve_pass(f, db, n, dx, n, 1, 0, nullptr, nullptr, nullptr))) return rc;
-    HIP_TRY(hipMemcpy(hprev.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if ((rc = solve_pass(f, db, n, dx, n, nr, 0, nullptr, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpy(hprev.data(), dx, sizeof(double) * len, hipMemcpyDeviceToHost));
     std::vector<double> c;
     int S = -1;
     for (int j = 1; j <= kMax && S < 0; j++) {
-        HIP_TRY(spike::launch_residual(n, f->d_ia, f->d_ja, f->d_a, db, n, dx, n, f->d_r, n, 1,
-                                       f->plan.flip ? 1 : 0, nullptr));
-        if ((rc = solve_pass(f, f->d_r, n, dx, n, 1, 1, nullptr, nullptr, nullptr))) return rc;
-        HIP_TRY(hipMemcpy(hx.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost));
-        double dm = 0.0, xm = 0.0;
-        for (int i = 0; i < n; i++) {
-            dm = std::max(dm, std::fabs(hx[i] - hprev[i]));
-            xm = std::max(xm, std::fabs(hx[i]));
+        if ((rc = residual(f, db, n, dx, n, nr, nullptr))) return rc;
+        if ((rc = solve_pass(f, f->d_r, n, dx, n, nr, 1, nullptr, nullptr, nullptr))) return rc;
+        HIP_TRY(hipMemcpy(hx.data(), dx, sizeof(double) * len, hipMemcpyDeviceToHost));
+        double worst = 0.0;
+        for (const auto &sg : seg) {
+            double dm = 0.0, xm = 0.0;
+            for (size_t i = sg.first; i < sg.second; i++) {
+                dm = std::max(dm, std::fabs(hx[i] - hprev[i]));
+                xm = std::max(xm, std::fabs(hx[i]));
+            }
+            const double cj = xm > 0 ? dm / xm : (dm > 0 ? HUGE_VAL : 0.0);
+            worst = std::max(worst, cj);
         }
-        c.push_back(xm > 0 ? dm / xm : 0.0);
-        if (!std::isfinite(c.back())) break;
-        if (c.back() <= tol) S = j - 1;
+        c.push_back(worst);
+        if (!std::isfinite(worst)) break;
+        if (worst <= tol) S = j - 1;
         hprev.swap(hx);
     }
-    dfree(f, db, sizeof(double) * n);
-    dfree(f, dx, sizeof(double) * n);
+    dfree(f, db, sizeof(double) * len);
+    dfree(f, dx, sizeof(double) * len);
     if (S < 0) {
         char msg[160];
         std::snprintf(msg, sizeof msg, "FP32 refinement does not reach %.1e in %d steps (last correction %.2e)", tol,
@@ -827,6 +899,431 @@ void legacy_solve(char uplo, int m, double *b, double *x, const char *who, bool 
     }
     int rc = factor_solve(g_cur, 1, b, m, x, m, nullptr, timed);
     if (rc != SPIKE_OK) std::fprintf(stderr, "%s: %s\n", who, t_err.c_str());
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- batches
+
+struct spike_batch_s {
+    char uplo = 'L';
+    int count = 0;
+    spike_batch_options_t opts{};
+    std::vector<int64_t> off;              // count+1 packed offsets
+    std::vector<spike_factor_s *> groups;  // block-diagonal factors
+    int device = 0;
+    int rpp[4] = {0, 0, 0, 0};             // rows per partition at 1, 2, 4, 8 RHS slots
+    double autotune_s = 0.0, preproc_s = 0.0;
+    // captured solve graph for one (b, x) pair
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const double *gb = nullptr;
+    double *gx = nullptr;
+    int graph_nodes = 0;
+    // host-pointer staging
+    double *d_sb = nullptr, *d_sx = nullptr;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        gb = nullptr;
+        gx = nullptr;
+    }
+    void release()
+    {
+        drop_graph();
+        for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        streams.clear();
+        events.clear();
+        if (d_sb) (void)hipFree(d_sb);
+        if (d_sx) (void)hipFree(d_sx);
+        d_sb = d_sx = nullptr;
+        for (spike_factor_s *f : groups) {
+            f->release();
+            delete f;
+        }
+        groups.clear();
+    }
+};
+
+namespace {
+
+struct GroupSpec {
+    int slots = 1, G = 1;
+    std::vector<int> mats;
+};
+
+// Basic per-matrix CSR checks needed before the matrix is copied into a
+// block-diagonal system (the full conventions are checked on that system).
+std::string check_batch_matrix(const spike_batch_matrix_t &m, int i)
+{
+    const std::string who = "matrix " + std::to_string(i) + ": ";
+    if (m.n < 1) return who + "n must be >= 1";
+    if (!m.a || !m.ia || !m.ja) return who + "a, ia and ja must be non-NULL";
+    if (m.nrhs < 1 || m.nrhs > 8) return who + "nrhs must be in [1, 8]";
+    if (m.ia[0] != 0) return who + "ia[0] must be 0";
+    for (int r = 0; r < m.n; r++)
+        if (m.ia[r + 1] <= m.ia[r]) return who + "row " + std::to_string(r) + " is empty (diagonal missing)";
+    for (int e = 0; e < m.ia[m.n]; e++)
+        if (m.ja[e] < 0 || m.ja[e] >= m.n) return who + "column index out of range";
+    return "";
+}
+
+// Builds one group: its matrices as one block-diagonal lower-form system
+// ('U' matrices reversed one by one), partitioned matrix by matrix with about
+// rpp rows per partition (the reference's formula within each matrix).
+int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpec &g, int rpp, int precision,
+                int refine_steps, spike_factor_s **out)
+{
+    *out = nullptr;
+    const bool flip = bt->uplo == 'U';
+    int64_t rows = 0, nnz = 0;
+    for (int i : g.mats) {
+        rows += m[i].n;
+        nnz += m[i].ia[m[i].n];
+    }
+    if (rows >= INT32_MAX || nnz > INT32_MAX) return fail(SPIKE_ERR_UNSUPPORTED, "batch group exceeds 2^31-1 entries");
+    std::vector<int> ia((size_t)rows + 1), ja((size_t)nnz), starts{0};
+    std::vector<double> a((size_t)nnz);
+    spike_factor_s *f = new spike_factor_s();
+    spike_options_default(&f->opts);
+    f->opts.precision = precision;
+    f->opts.lanes_per_row = g.G;
+    f->opts.drop_tol = bt->opts.drop_tol;
+    f->opts.refine_steps = refine_steps;
+    f->opts.refine_tol = bt->opts.refine_tol;
+    f->batch_flip = flip;
+    f->batch_slots = g.slots;
+    f->vec_len = bt->off[bt->count];
+    const size_t vb = precision == SPIKE_PREC_FP32_REFINED ? sizeof(float) : sizeof(double);
+    const int kcap = std::min(kmax_limit_for_device(), (int)(spike::block_solve_lds_limit() / (vb * g.slots)) - 2);
+    int64_t r0 = 0;
+    ia[0] = 0;
+    for (int i : g.mats) {
+        const spike_batch_matrix_t &M = m[i];
+        const int n = M.n;
+        for (int r = 0; r < n; r++) {
+            const int src = flip ? n - 1 - r : r;
+            const int b0 = M.ia[src], b1 = M.ia[src + 1];
+            int64_t p = ia[r0 + r];
+            if (!flip)
+                for (int e = b0; e < b1; e++, p++) {
+                    ja[p] = (int)r0 + M.ja[e];
+                    a[p] = M.a[e];
+                }
+            else
+                for (int e = b1 - 1; e >= b0; e--, p++) {
+                    ja[p] = (int)r0 + (n - 1 - M.ja[e]);
+                    a[p] = M.a[e];
+                }
+            ia[r0 + r + 1] = (int)p;
+        }
+        f->mats.push_back(spike_factor_s::Mat{(int)r0, n, M.nrhs, bt->off[i]});
+        f->batch_max_nrhs = std::max(f->batch_max_nrhs, M.nrhs);
+        int P = std::max(1, (n + rpp / 2) / std::max(rpp, 1));
+        P = std::min(P, n);
+        while (P < n && n - (n / P) * (P - 1) > kcap) P++;
+        for (int t = 1; t < P; t++) starts.push_back((int)r0 + (n / P) * t);
+        starts.push_back((int)(r0 + n));
+        r0 += n;
+    }
+    int rc = factor_build(f, 'L', (int)rows, a.data(), ia.data(), ja.data(), &starts);
+    if (rc != SPIKE_OK) {
+        std::string keep = t_err;
+        f->release();
+        delete f;
+        t_err = keep;
+        return rc;
+    }
+    *out = f;
+    return SPIKE_OK;
+}
+
+// Groups: right-hand-side slots (next power of two >= nrhs) x lanes per row
+// (from the p90 off-diagonal row length, or the option); a group is split
+// where it would pass 1.5e9 entries.
+std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vector<int> &idx, int lanes)
+{
+    std::vector<GroupSpec> out;
+    std::vector<std::pair<int, int>> key(idx.size());
+    for (size_t j = 0; j < idx.size(); j++) {
+        const spike_batch_matrix_t &M = m[idx[j]];
+        int slots = 1;
+        while (slots < M.nrhs) slots <<= 1;
+        int G = lanes;
+        if (G <= 0) {
+            std::vector<int> len(M.n);
+            for (int r = 0; r < M.n; r++) len[r] = M.ia[r + 1] - M.ia[r] - 1;
+            G = spike::choose_lanes(len);
+        }
+        key[j] = {slots, G};
+    }
+    std::vector<std::pair<int, int>> keys(key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (const auto &k : keys) {
+        GroupSpec cur;
+        cur.slots = k.first;
+        cur.G = k.second;
+        int64_t nnz = 0;
+        for (size_t j = 0; j < idx.size(); j++) {
+            if (key[j] != k) continue;
+            const int64_t e = m[idx[j]].ia[m[idx[j]].n];
+            if (!cur.mats.empty() && nnz + e > 1500000000) {
+                out.push_back(cur);
+                cur.mats.clear();
+                nnz = 0;
+            }
+            cur.mats.push_back(idx[j]);
+            nnz += e;
+        }
+        if (!cur.mats.empty()) out.push_back(cur);
+    }
+    return out;
+}
+
+int batch_run(spike_batch_s *bt, const double *b, double *x, hipStream_t s)
+{
+    for (spike_factor_s *f : bt->groups) {
+        int rc = solve_chunk(f, b, f->plan.n, x, f->plan.n, f->batch_slots, s, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return SPIKE_OK;
+}
+
+// Builds the groups of matrices idx (packed offsets already in bt->off).
+int batch_build(spike_batch_s *bt, const spike_batch_matrix_t *m, const std::vector<int> &idx, const int *rpp,
+                int refine_steps)
+{
+    std::vector<GroupSpec> specs = plan_groups(m, idx, bt->opts.lanes_per_row);
+    for (const GroupSpec &g : specs) {
+        spike_factor_s *f = nullptr;
+        int rc = build_group(bt, m, g, rpp[__builtin_ctz((unsigned)g.slots)], bt->opts.precision, refine_steps,
+                             &f);
+        if (rc) return rc;
+        bt->groups.push_back(f);
+    }
+    if (bt->opts.precision == SPIKE_PREC_FP32_REFINED && !bt->groups.empty()) {
+        // one residual workspace for all groups: their matrices own disjoint
+        // ranges of the packed layout
+        spike_factor_s *f0 = bt->groups[0];
+        int rc = f0->dalloc(&f0->d_r, (size_t)bt->off[bt->count]);
+        if (rc) return rc;
+        for (spike_factor_s *f : bt->groups) f->d_r = f0->d_r;
+    }
+    return SPIKE_OK;
+}
+
+// Refinement steps of an FP32 batch: calibrate_refinement's procedure over
+// every matrix of every group at once.
+int batch_calibrate(spike_batch_s *bt)
+{
+    const double tol = bt->opts.refine_tol > 0 ? bt->opts.refine_tol : 1e-13;
+    constexpr int kMax = 8;
+    const size_t len = (size_t)bt->off[bt->count];
+    std::vector<double> hb(len), hx(len), hprev(len);
+    uint64_t st = 0x5350494B45434131ull;
+    for (size_t i = 0; i < len; i++) {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        hb[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+    }
+    double *db = nullptr, *dx = nullptr;
+    HIP_TRY(hipMalloc((void **)&db, sizeof(double) * len));
+    hipError_t e = hipMalloc((void **)&dx, sizeof(double) * len);
+    if (e != hipSuccess) {
+        (void)hipFree(db);
+        return fail(SPIKE_ERR_NOMEM, "calibration buffers");
+    }
+    auto done = [&](int rc) {
+        (void)hipFree(db);
+        (void)hipFree(dx);
+        return rc;
+    };
+    if (hipMemcpy(db, hb.data(), sizeof(double) * len, hipMemcpyHostToDevice) != hipSuccess)
+        return done(fail(SPIKE_ERR_DEVICE, "calibration upload"));
+    for (spike_factor_s *f : bt->groups) f->refine_steps = 0;
+    int rc = batch_run(bt, db, dx, nullptr);
+    if (rc) return done(rc);
+    if (hipMemcpy(hprev.data(), dx, sizeof(double) * len, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(fail(SPIKE_ERR_DEVICE, "calibration download"));
+    std::vector<double> c;
+    int S = -1;
+    for (int j = 1; j <= kMax && S < 0; j++) {
+        for (spike_factor_s *f : bt->groups) {
+            if ((rc = residual(f, db, f->plan.n, dx, f->plan.n, f->batch_slots, nullptr))) return done(rc);
+            if ((rc = solve_pass(f, f->d_r, f->plan.n, dx, f->plan.n, f->batch_slots, 1, nullptr, nullptr, nullptr)))
+                return done(rc);
+        }
+        if (hipMemcpy(hx.data(), dx, sizeof(double) * len, hipMemcpyDeviceToHost) != hipSuccess)
+            return done(fail(SPIKE_ERR_DEVICE, "calibration download"));
+        double worst = 0.0;
+        for (int i = 0; i < bt->count; i++) {
+            double dm = 0.0, xm = 0.0;
+            for (int64_t q = bt->off[i]; q < bt->off[i + 1]; q++) {
+                dm = std::max(dm, std::fabs(hx[q] - hprev[q]));
+                xm = std::max(xm, std::fabs(hx[q]));
+            }
+            worst = std::max(worst, xm > 0 ? dm / xm : (dm > 0 ? HUGE_VAL : 0.0));
+        }
+        c.push_back(worst);
+        if (!std::isfinite(worst)) break;
+        if (worst <= tol) S = j - 1;
+        hprev.swap(hx);
+    }
+    if (S < 0) return done(fail(SPIKE_ERR_UNSUPPORTED, "FP32 refinement of the batch does not converge"));
+    for (spike_factor_s *f : bt->groups) {
+        f->refine_steps = S;
+        f->refine_est = c[S];
+        f->stats.refine_steps = S;
+        f->stats.refine_est_err = c[S];
+    }
+    return done(SPIKE_OK);
+}
+
+// Autotuner for the partition size, per right-hand-side slot class (1, 2,
+// 4, 8 slots).  Block solves are latency-bound, so their speed depends on how
+// many partitions each CU keeps resident, which the LDS x segments (rows x
+// slots x value size) limit; the reduced system grows as partitions shrink.
+// For every class present, a sample of the class's matrices (every j-th,
+// about 1024 x the largest candidate rows, enough to fill the GPU) is built
+// with each candidate size and timed over three whole solves; the candidate
+// with the most rows x RHS per second wins.
+int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
+{
+    const int scale = bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 2 : 1;
+    static const int cand[4][3] = {{512, 1024, 2048}, {256, 512, 1024}, {256, 512, 1024}, {128, 256, 512}};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = SPIKE_OK;
+    for (int li = 0; li < 4 && !rc; li++) {
+        best[li] = cand[li][1] * scale;
+        std::vector<int> members;
+        int64_t rows = 0;
+        for (int i = 0; i < bt->count; i++) {
+            int slots = 1;
+            while (slots < m[i].nrhs) slots <<= 1;
+            if (slots == (1 << li)) {
+                members.push_back(i);
+                rows += m[i].n;
+            }
+        }
+        if (members.empty()) continue;
+        const int64_t target = 1024LL * cand[li][2] * scale;
+        const int64_t want = bt->opts.autotune_sample > 0 ? bt->opts.autotune_sample : INT64_MAX;
+        const size_t step = rows > target ? (size_t)(rows / target) : 1;
+        std::vector<spike_batch_matrix_t> sm;
+        for (size_t j = 0; j < members.size() && (int64_t)sm.size() < want; j += std::max<size_t>(step, 1))
+            sm.push_back(m[members[j]]);
+        std::vector<int> sidx(sm.size());
+        spike_batch_s tmp;
+        tmp.uplo = bt->uplo;
+        tmp.count = (int)sm.size();
+        tmp.opts = bt->opts;
+        tmp.off.assign(sm.size() + 1, 0);
+        for (size_t j = 0; j < sm.size(); j++) {
+            sidx[j] = (int)j;
+            tmp.off[j + 1] = tmp.off[j] + (int64_t)sm[j].n * sm[j].nrhs;
+        }
+        const size_t len = (size_t)tmp.off[sm.size()];
+        double *db = nullptr, *dx = nullptr;
+        if (hipMalloc((void **)&db, sizeof(double) * len) != hipSuccess ||
+            hipMalloc((void **)&dx, sizeof(double) * len) != hipSuccess) {
+            (void)hipGetLastError();
+            if (db) (void)hipFree(db);
+            rc = fail(SPIKE_ERR_NOMEM, "autotune buffers");
+            break;
+        }
+        std::vector<double> hb(len);
+        for (size_t i = 0; i < len; i++) hb[i] = std::sin(0.37 * (double)i);
+        (void)hipMemcpy(db, hb.data(), sizeof(double) * len, hipMemcpyHostToDevice);
+        double best_rate = -1.0;
+        for (int c0 : cand[li]) {
+            const int c = c0 * scale;
+            int r4[4] = {c, c, c, c};
+            tmp.release();
+            if (batch_build(&tmp, sm.data(), sidx, r4, bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 1 : 0)) {
+                t_err.clear();
+                continue;   // candidate does not fit: skip it
+            }
+            if ((rc = batch_run(&tmp, db, dx, nullptr))) break;
+            (void)hipEventRecord(e0, nullptr);
+            for (int r = 0; r < 3 && !rc; r++) rc = batch_run(&tmp, db, dx, nullptr);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            if (rc) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double rate = ms > 0 ? 3.0 * (double)len / (ms * 1e-3) : 0.0;
+            if (rate > best_rate) {
+                best_rate = rate;
+                best[li] = c;
+            }
+        }
+        tmp.release();
+        (void)hipFree(db);
+        (void)hipFree(dx);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Captures every group's launch sequence into one graph: group i on branch
+// i % B of B capture streams forked from and joined back to the first.
+int batch_capture(spike_batch_s *bt, const double *b, double *x)
+{
+    bt->drop_graph();
+    const int B = std::max(1, std::min(4, (int)bt->groups.size()));
+    while ((int)bt->streams.size() < B) {
+        hipStream_t s;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        bt->streams.push_back(s);
+    }
+    while ((int)bt->events.size() < B) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        bt->events.push_back(e);
+    }
+    hipStream_t s0 = bt->streams[0];
+    HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+    int rc = SPIKE_OK;
+    hipError_t e = hipEventRecord(bt->events[0], s0);
+    for (int i = 1; i < B && e == hipSuccess; i++) e = hipStreamWaitEvent(bt->streams[i], bt->events[0], 0);
+    for (size_t g = 0; g < bt->groups.size() && e == hipSuccess && !rc; g++) {
+        spike_factor_s *f = bt->groups[g];
+        rc = solve_chunk(f, b, f->plan.n, x, f->plan.n, f->batch_slots, bt->streams[g % B], nullptr, nullptr);
+    }
+    for (int i = 1; i < B && e == hipSuccess && !rc; i++) {
+        e = hipEventRecord(bt->events[i], bt->streams[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s0, bt->events[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s0, &g);
+    if (rc || e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (rc) return rc;
+        return fail(SPIKE_ERR_DEVICE, std::string("graph capture failed: ") +
+                                          hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    bt->graph = g;
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(g, nullptr, &nodes);
+    bt->graph_nodes = (int)nodes;
+    HIP_TRY(hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
+    bt->gb = b;
+    bt->gx = x;
+    return SPIKE_OK;
 }
 
 }  // namespace
@@ -1006,6 +1503,149 @@ void _spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthre
 {
     (void)nthreads;
     legacy_solve(uplo, m, b, x, "_spike_pstrsv_solve", true);
+}
+
+// ---------------------------------------------------------------- batch API
+
+void spike_batch_options_default(spike_batch_options_t *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->rows_per_partition = 0;
+    o->lanes_per_row = 0;
+    o->precision = SPIKE_PREC_FP64;
+    o->use_graph = 1;
+    o->autotune_sample = 0;
+    o->drop_tol = -1.0;
+    o->refine_tol = 0.0;
+}
+
+int spike_batch_create(spike_batch_t *out, char uplo, int count, const spike_batch_matrix_t *m,
+                       const spike_batch_options_t *opts)
+{
+    if (!out) return fail(SPIKE_ERR_ARG, "batch pointer is NULL");
+    *out = nullptr;
+    if (uplo == 'l') uplo = 'L';
+    if (uplo == 'u') uplo = 'U';
+    if (uplo != 'L' && uplo != 'U') return fail(SPIKE_ERR_ARG, "uplo must be L or U");
+    if (count < 1 || !m) return fail(SPIKE_ERR_ARG, "count must be >= 1 and m non-NULL");
+    spike_batch_options_t o;
+    spike_batch_options_default(&o);
+    if (opts) o = *opts;
+    if (o.precision != SPIKE_PREC_FP64 && o.precision != SPIKE_PREC_FP32_REFINED)
+        return fail(SPIKE_ERR_ARG, "unknown precision mode");
+    for (int i = 0; i < count; i++) {
+        std::string err = check_batch_matrix(m[i], i);
+        if (!err.empty()) return fail(SPIKE_ERR_MATRIX, err);
+    }
+    const double t0 = now_s();
+    spike_batch_s *bt = new spike_batch_s();
+    bt->uplo = uplo;
+    bt->count = count;
+    bt->opts = o;
+    (void)hipGetDevice(&bt->device);
+    bt->off.assign(count + 1, 0);
+    for (int i = 0; i < count; i++) bt->off[i + 1] = bt->off[i] + (int64_t)m[i].n * m[i].nrhs;
+    auto bail = [&](int rc) {
+        std::string keep = t_err;
+        bt->release();
+        delete bt;
+        t_err = keep;
+        return rc;
+    };
+    int rc = SPIKE_OK;
+    for (int li = 0; li < 4; li++) bt->rpp[li] = o.rows_per_partition;
+    if (o.rows_per_partition <= 0) {
+        const double ta = now_s();
+        if ((rc = autotune_rpp(bt, m, bt->rpp))) return bail(rc);
+        bt->autotune_s = now_s() - ta;
+    }
+    std::vector<int> idx(count);
+    for (int i = 0; i < count; i++) idx[i] = i;
+    if ((rc = batch_build(bt, m, idx, bt->rpp, 0))) return bail(rc);
+    if (o.precision == SPIKE_PREC_FP32_REFINED && (rc = batch_calibrate(bt))) return bail(rc);
+    bt->preproc_s = now_s() - t0;
+    *out = bt;
+    t_err.clear();
+    return SPIKE_OK;
+}
+
+int64_t spike_batch_offset(spike_batch_t bt, int i)
+{
+    if (!bt || i < 0 || i > bt->count) return -1;
+    return bt->off[i];
+}
+
+int64_t spike_batch_length(spike_batch_t bt) { return bt ? bt->off[bt->count] : -1; }
+
+int spike_batch_solve(spike_batch_t bt, const double *b, double *x, void *stream)
+{
+    if (!bt) return fail(SPIKE_ERR_STATE, "batch is NULL");
+    if (!b || !x) return fail(SPIKE_ERR_ARG, "b and x must be non-NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != bt->device) HIP_TRY(hipSetDevice(bt->device));
+    const size_t len = (size_t)bt->off[bt->count];
+    const bool bdev = is_device_ptr(b), xdev = is_device_ptr(x);
+    const double *db = b;
+    double *dx = x;
+    if (!bdev || !xdev) {
+        if (!bt->d_sb) {
+            HIP_TRY(hipMalloc((void **)&bt->d_sb, sizeof(double) * len));
+            HIP_TRY(hipMalloc((void **)&bt->d_sx, sizeof(double) * len));
+        }
+        if (!bdev) {
+            HIP_TRY(hipMemcpyAsync(bt->d_sb, b, sizeof(double) * len, hipMemcpyHostToDevice, s));
+            db = bt->d_sb;
+        }
+        if (!xdev) dx = bt->d_sx;
+    }
+    int rc;
+    if (bt->opts.use_graph) {
+        if (!bt->exec || bt->gb != db || bt->gx != dx) {
+            // the capture streams must not overlap earlier work on s
+            HIP_TRY(hipStreamSynchronize(s));
+            if ((rc = batch_capture(bt, db, dx))) return rc;
+        }
+        HIP_TRY(hipGraphLaunch(bt->exec, s));
+    } else if ((rc = batch_run(bt, db, dx, s))) {
+        return rc;
+    }
+    if (!xdev) HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+    if (!bdev || !xdev) HIP_TRY(hipStreamSynchronize(s));
+    return SPIKE_OK;
+}
+
+int spike_batch_stats(spike_batch_t bt, spike_batch_stats_t *st)
+{
+    if (!bt || !st) return fail(SPIKE_ERR_ARG, "NULL argument");
+    std::memset(st, 0, sizeof *st);
+    st->count = bt->count;
+    st->groups = (int)bt->groups.size();
+    st->rows_rhs = bt->off[bt->count];
+    for (spike_factor_s *f : bt->groups) {
+        st->rows += f->plan.n;
+        st->nnz += f->stats.nnz;
+        st->partitions += f->plan.P;
+        st->n_reduced += f->stats.n_reduced;
+        st->device_bytes += f->device_bytes;
+        st->refine_steps = std::max(st->refine_steps, f->refine_steps);
+        st->refine_est_err = std::max(st->refine_est_err, f->refine_est);
+    }
+    for (int li = 0; li < 4; li++) st->rows_per_partition[li] = bt->rpp[li];
+    st->preproc_seconds = bt->preproc_s;
+    st->autotune_seconds = bt->autotune_s;
+    st->graph_nodes = bt->graph_nodes;
+    return SPIKE_OK;
+}
+
+int spike_batch_destroy(spike_batch_t bt)
+{
+    if (!bt) return SPIKE_OK;
+    bt->release();
+    delete bt;
+    return SPIKE_OK;
 }
 
 }  // extern "C"

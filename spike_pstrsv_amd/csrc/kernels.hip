@@ -120,6 +120,36 @@ __device__ __forceinline__ void put_x(double *x, size_t i, double v, int accum)
     x[i] = accum ? x[i] + v : v;
 }
 
+// Caller-side addressing of one partition's right-hand sides and solutions.
+struct RhsMap {
+    const double *b;
+    double *x;
+    int ldb, ldx, nr, row0, mn, mflip;
+    // caller index of lower-form row g
+    __device__ __forceinline__ int at(int g) const
+    {
+        const int l = g - row0;
+        return mflip ? mn - 1 - l : l;
+    }
+    __device__ __forceinline__ double bv(int r, int g) const
+    {
+        return b[(size_t)(r < nr ? r : nr - 1) * ldb + at(g)];
+    }
+};
+__device__ __forceinline__ RhsMap rhs_map(const BlockArgs &A, const DevPart &pt)
+{
+    RhsMap m;
+    m.b = A.b + pt.boff;
+    m.x = A.x + pt.boff;
+    m.ldb = pt.ld ? pt.ld : A.ldb;
+    m.ldx = pt.ld ? pt.ld : A.ldx;
+    m.nr = pt.mnr ? pt.mnr : A.nr;
+    m.row0 = pt.row0;
+    m.mn = pt.mn;
+    m.mflip = pt.mflip;
+    return m;
+}
+
 // Pass-2 right-hand side of one partition in LDS, NR right-hand sides
 // interleaved (xs[i*NR + r]): b for every row, then b - sum_j c_ij xr[col_j]
 // on the coupling rows (times 1/diag when scaled).  The coupling products are
@@ -129,15 +159,13 @@ __device__ __forceinline__ void put_x(double *x, size_t i, double v, int accum)
 template <int NR, class T>
 __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, T *prod, bool scaled)
 {
-    const int lane = threadIdx.x, s = pt.s, k = pt.k, n = A.n, flip = A.flip;
+    const int lane = threadIdx.x, s = pt.s, k = pt.k;
+    const RhsMap m = rhs_map(A, pt);
     for (int i = lane; i < k; i += 64) {
-        const int g = s + i, gm = flip ? n - 1 - g : g;
+        const int g = s + i;
         const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int rr = r < A.nr ? r : A.nr - 1;
-            xs[i * NR + r] = (T)A.b[(size_t)rr * A.ldb + gm] * sc;
-        }
+        for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(r, g) * sc;
     }
     if (pt.ccnt == 0) return;
     if (A.cpl_fused) {
@@ -152,7 +180,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
         }
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
-            const int g = A.cpl_rows[pt.cbeg + c], gm = flip ? n - 1 - g : g;
+            const int g = A.cpl_rows[pt.cbeg + c];
             const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
             T acc[NR];
 #pragma unroll
@@ -161,10 +189,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
 #pragma unroll
                 for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
 #pragma unroll
-            for (int r = 0; r < NR; r++) {
-                const int rr = r < A.nr ? r : A.nr - 1;
-                xs[(g - s) * NR + r] = ((T)A.b[(size_t)rr * A.ldb + gm] - acc[r]) * sc;
-            }
+            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = ((T)m.bv(r, g) - acc[r]) * sc;
         }
     } else {
         __syncthreads();
@@ -194,17 +219,13 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     // (then its pass-1 result is final); pass 2 only when it has coupling.
     if (PASS == 1 && pt.hi1 < 0) return;
     if (PASS == 2 && pt.f >= k) return;
-    const int n = A.n, flip = A.flip;
+    const RhsMap m = rhs_map(A, pt);
 
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
         for (int i = lane; i < k; i += 64) {
-            const int g = s + i, gm = flip ? n - 1 - g : g;
 #pragma unroll
-            for (int r = 0; r < NR; r++) {
-                const int rr = r < A.nr ? r : A.nr - 1;
-                xs[i * NR + r] = (T)A.b[(size_t)rr * A.ldb + gm];
-            }
+            for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(r, s + i);
         }
     } else {
         pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false);
@@ -310,10 +331,10 @@ epilogue:
     // pass 2 writes the solution; without coupling, pass 1's result is final
     if (PASS == 2 || pt.f >= k)
         for (int i = lane; i < k; i += 64) {
-            const int g = s + i, gm = flip ? n - 1 - g : g;
+            const int gm = m.at(s + i);
 #pragma unroll
             for (int r = 0; r < NR; r++)
-                if (r < A.nr) put_x(A.x, (size_t)r * A.ldx + gm, (double)xs[i * NR + r], A.accum);
+                if (r < m.nr) put_x(m.x, (size_t)r * m.ldx + gm, (double)xs[i * NR + r], A.accum);
         }
 }
 
@@ -337,13 +358,13 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
     const int s = pt.s, k = pt.k;
     if (PASS == 1 && pt.hi1 < 0) return;
     if (PASS == 2 && pt.f >= k) return;
-    const int n = A.n, flip = A.flip;
     const double *__restrict__ invd = A.inv_diag;
+    const RhsMap m = rhs_map(A, pt);
 
     if (PASS == 1) {
         for (int r = lane; r < k; r += 64) {
             const int g = s + r;
-            xs[r] = A.b[flip ? n - 1 - g : g] * invd[g];
+            xs[r] = m.bv(0, g) * invd[g];
         }
     } else {
         pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + 2, true);
@@ -424,10 +445,7 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
     if (PASS == 1)
         for (int q = lane; q < pt.dcnt; q += 64) A.xred[pt.dbeg + q] = xs[A.dep_rows[pt.dbeg + q] - s];
     if (PASS == 2 || pt.f >= k)
-        for (int r = lane; r < k; r += 64) {
-            const int g = s + r;
-            put_x(A.x, flip ? n - 1 - g : g, xs[r], A.accum);
-        }
+        for (int r = lane; r < k; r += 64) put_x(m.x, m.at(s + r), xs[r], A.accum);
 }
 
 template <int G, int D, int PASS, int NR, class T>
@@ -594,66 +612,74 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
 
 // ---------------------------------------------------------------- coupling
 
+// One workgroup per partition: threads over its coupling rows.
 template <int NR>
-__global__ __launch_bounds__(256) void k_coupling(int ncpl, const int *__restrict__ rows,
+__global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ parts, const int *__restrict__ rows,
                                                   const int *__restrict__ ptr, const int *__restrict__ col,
                                                   const double *__restrict__ val, const double *__restrict__ xred,
-                                                  const double *__restrict__ b, int ldb, int nr,
-                                                  double *__restrict__ bc, int n, int flip)
+                                                  BlockArgs A, double *__restrict__ bc)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncpl) return;
-    const int g = rows[c], gm = flip ? n - 1 - g : g;
-    double acc[NR];
+    const DevPart pt = parts[blockIdx.x];
+    const RhsMap m = rhs_map(A, pt);
+    for (int c = pt.cbeg + threadIdx.x; c < pt.cbeg + pt.ccnt; c += blockDim.x) {
+        const int g = rows[c];
+        double acc[NR];
 #pragma unroll
-    for (int t = 0; t < NR; t++) acc[t] = 0.0;
-    for (int j = ptr[c]; j < ptr[c + 1]; j++) {
-        const double w = val[j];
-        const size_t x0 = (size_t)col[j] * NR;
+        for (int t = 0; t < NR; t++) acc[t] = 0.0;
+        for (int j = ptr[c]; j < ptr[c + 1]; j++) {
+            const double w = val[j];
+            const size_t x0 = (size_t)col[j] * NR;
 #pragma unroll
-        for (int t = 0; t < NR; t++) acc[t] = fma(w, xred[x0 + t], acc[t]);
-    }
+            for (int t = 0; t < NR; t++) acc[t] = fma(w, xred[x0 + t], acc[t]);
+        }
 #pragma unroll
-    for (int t = 0; t < NR; t++) {
-        const int tt = t < nr ? t : nr - 1;
-        bc[(size_t)c * NR + t] = b[(size_t)tt * ldb + gm] - acc[t];
+        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = m.bv(t, g) - acc[t];
     }
 }
 
 // ---------------------------------------------------------------- residual
 
-// r = b - A x in FP64, G = 8 lanes per lower-form row (strided over its
-// entries), NR right-hand sides per pass (nr active).  Lane sums are combined
-// with DPP in a fixed order.
+// r = b - A x in FP64, one workgroup per partition, G = 8 lanes per
+// lower-form row (strided over its entries), NR right-hand sides (nr of the
+// partition's matrix active).  r uses the same caller addressing as b (ldr
+// replaces ldb when the partition has no leading dimension of its own).
+// Lane sums are combined with DPP in a fixed order.
 template <int NR>
-__global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                                  const double *__restrict__ a, const double *__restrict__ b,
-                                                  int ldb, const double *__restrict__ x, int ldx,
-                                                  double *__restrict__ r, int ldr, int nr, int flip)
+__global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ parts, const int *__restrict__ ia,
+                                                  const int *__restrict__ ja, const double *__restrict__ a,
+                                                  BlockArgs A, double *__restrict__ r, int ldr, int c0)
 {
     constexpr int G = 8;
-    const int gid = blockIdx.x * 256 + threadIdx.x;
-    const int row = gid / G, li = gid % G;
-    double acc[NR];
+    const DevPart pt = parts[blockIdx.x];
+    const RhsMap m = rhs_map(A, pt);
+    const int ld_r = pt.ld ? pt.ld : ldr;
+    double *rp = r + pt.boff;
+    const int li = threadIdx.x % G;
+    for (int base = 0; base < pt.k; base += 256 / G) {
+        const int lr = base + threadIdx.x / G;
+        const int row = pt.s + lr;
+        double acc[NR];
 #pragma unroll
-    for (int t = 0; t < NR; t++) acc[t] = 0.0;
-    if (row < n) {
-        const int e1 = ia[row + 1];
-        for (int e = ia[row] + li; e < e1; e += G) {
-            const double v = a[e];
-            const int c = ja[e], cm = flip ? n - 1 - c : c;
+        for (int t = 0; t < NR; t++) acc[t] = 0.0;
+        if (lr < pt.k) {
+            const int e1 = ia[row + 1];
+            for (int e = ia[row] + li; e < e1; e += G) {
+                const double v = a[e];
+                const int cm = m.at(ja[e]);
+#pragma unroll
+                for (int t = 0; t < NR; t++)
+                    if (c0 + t < m.nr) acc[t] = fma(v, m.x[(size_t)(c0 + t) * m.ldx + cm], acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
+        if (lr < pt.k && li == 0) {
+            const int gm = m.at(row);
 #pragma unroll
             for (int t = 0; t < NR; t++)
-                if (t < nr) acc[t] = fma(v, x[(size_t)t * ldx + cm], acc[t]);
+                if (c0 + t < m.nr)
+                    rp[(size_t)(c0 + t) * ld_r + gm] = m.b[(size_t)(c0 + t) * m.ldb + gm] - acc[t];
         }
-    }
-#pragma unroll
-    for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
-    if (row < n && li == 0) {
-        const int gm = flip ? n - 1 - row : row;
-#pragma unroll
-        for (int t = 0; t < NR; t++)
-            if (t < nr) r[(size_t)t * ldr + gm] = b[(size_t)t * ldb + gm] - acc[t];
     }
 }
 
@@ -823,36 +849,31 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
     return hipGetLastError();
 }
 
-hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *cpl_val, const double *xred, const double *b, int ldb, int nr,
-                           int nr_slots, double *bc, int n, int flip, hipStream_t s)
+hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
+                           const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
 {
-    if (ncpl <= 0) return hipSuccess;
-    const dim3 grid((ncpl + 255) / 256), block(256);
-    switch (nr_slots) {
-    case 1: hipLaunchKernelGGL(k_coupling<1>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
-    case 2: hipLaunchKernelGGL(k_coupling<2>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
-    case 4: hipLaunchKernelGGL(k_coupling<4>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
-    case 8: hipLaunchKernelGGL(k_coupling<8>, grid, block, 0, s, ncpl, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, b, ldb, nr, bc, n, flip); break;
+    if (P <= 0) return hipSuccess;
+    const dim3 grid(P), block(256);
+    switch (a.nr_slots) {
+    case 1: hipLaunchKernelGGL(k_coupling<1>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
+    case 2: hipLaunchKernelGGL(k_coupling<2>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
+    case 4: hipLaunchKernelGGL(k_coupling<4>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
+    case 8: hipLaunchKernelGGL(k_coupling<8>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_residual(int n, const int *ia, const int *ja, const double *a, const double *b, int ldb,
-                           const double *x, int ldx, double *r, int ldr, int nr, int flip, hipStream_t s)
+hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int *ja, const double *a_,
+                           const BlockArgs &a, int nr_max, double *r, int ldr, hipStream_t s)
 {
-    if (n <= 0 || nr <= 0) return hipSuccess;
-    const dim3 grid((unsigned)(((size_t)n * 8 + 255) / 256)), block(256);
-    for (int j = 0; j < nr; j += 4) {
-        const int c = nr - j < 4 ? nr - j : 4;
-        const double *bj = b + (size_t)j * ldb;
-        const double *xj = x + (size_t)j * ldx;
-        double *rj = r + (size_t)j * ldr;
-        if (c == 1)
-            hipLaunchKernelGGL(k_residual<1>, grid, block, 0, s, n, ia, ja, a, bj, ldb, xj, ldx, rj, ldr, c, flip);
+    if (P <= 0 || nr_max <= 0) return hipSuccess;
+    const dim3 grid(P), block(256);
+    for (int c0 = 0; c0 < nr_max; c0 += 4) {
+        if (nr_max - c0 == 1)
+            hipLaunchKernelGGL(k_residual<1>, grid, block, 0, s, parts, ia, ja, a_, a, r, ldr, c0);
         else
-            hipLaunchKernelGGL(k_residual<4>, grid, block, 0, s, n, ia, ja, a, bj, ldb, xj, ldx, rj, ldr, c, flip);
+            hipLaunchKernelGGL(k_residual<4>, grid, block, 0, s, parts, ia, ja, a_, a, r, ldr, c0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -13,6 +13,14 @@ struct DevPart {
     int hi1;        // < 0: the partition needs no pass 1
     int dbeg, dcnt; // dependents (reduced indices dbeg .. dbeg+dcnt)
     int cbeg, ccnt; // coupling rows (indices into the coupling arrays)
+    // Right-hand-side addressing of the matrix the partition belongs to (a
+    // factor may hold many matrices as one block-diagonal system).  Row g of
+    // the lower form is entry l = g - row0 of that matrix, stored at caller
+    // index (mflip ? mn-1-l : l); its right-hand sides and solutions start at
+    // b + boff / x + boff with leading dimension ld and mnr columns (ld == 0 /
+    // mnr == 0: the launch's ldb, ldx and nr).
+    int row0, mn, mflip, ld, mnr, pad_;
+    int64_t boff;
 };
 
 // Device view of a StepStream.
@@ -83,10 +91,10 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
                                const double *xin, double *xout, int nr_slots, hipStream_t s);
 
 // bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c and
-// right-hand side (bc, xred interleaved with nr_slots values per index).
-hipError_t launch_coupling(int ncpl, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *cpl_val, const double *xred, const double *b, int ldb, int nr,
-                           int nr_slots, double *bc, int n, int flip, hipStream_t s);
+// right-hand side (bc, xred interleaved with a.nr_slots values per index); b
+// is addressed per partition as in the block solve (a.b, a.ldb, a.nr).
+hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
+                           const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s);
 
 // Spike build: for every partition with kcnt > 0, W_p = D0^-1 C[:, K_p] on
 // rows [f, g] (row-major, ld = kcnt) using the lower-form CSR (ia, ja, a).
@@ -107,11 +115,12 @@ hipError_t launch_reduced_fill(int nred, const int64_t *wrow, const int *kbeg, c
                                const double *tol, const double *W, const int *kridx,
                                const int *rp, int *ci, double *v, hipStream_t s);
 
-// r = b - A x in FP64 over the lower-form CSR (ia, ja, a) for nr right-hand
-// sides (columns of b, x, r with leading dimensions ldb, ldx, ldr); with flip,
-// the caller's index of lower-form row i is n-1-i.
-hipError_t launch_residual(int n, const int *ia, const int *ja, const double *a, const double *b, int ldb,
-                           const double *x, int ldx, double *r, int ldr, int nr, int flip, hipStream_t s);
+// r = b - A x in FP64 over the lower-form CSR (ia, ja, a_) for the
+// right-hand sides of a.b / a.x (partition addressing as in the block solve;
+// nr_max = most columns of any partition); r has b's layout, with ldr where
+// a partition has no leading dimension of its own.
+hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int *ja, const double *a_,
+                           const BlockArgs &a, int nr_max, double *r, int ldr, hipStream_t s);
 
 // Largest dynamic LDS a block-solve workgroup may use, in bytes.
 size_t block_solve_lds_limit();

@@ -215,8 +215,13 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     if (uplo != 'L' && uplo != 'U') return "uplo must be 'L' or 'U'";
     if (n < 1) return "n must be >= 1";
     if (!a || !ia || !ja) return "a, ia and ja must be non-NULL";
-    int P = opt.P;
+    int P = opt.starts.empty() ? opt.P : (int)opt.starts.size() - 1;
     if (P < 1 || P > n) return "partition count must be in [1, n]";
+    if (!opt.starts.empty()) {
+        if (opt.starts[0] != 0 || opt.starts[P] != n) return "partition starts must begin at 0 and end at n";
+        for (int t = 0; t < P; t++)
+            if (opt.starts[t + 1] <= opt.starts[t]) return "partition starts must be strictly increasing";
+    }
     std::string err = validate(uplo, n, ia, ja);
     if (!err.empty()) return err;
     const int64_t nnz = ia[n];
@@ -255,12 +260,19 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     const int *LJ = pl.ja.data();
     const double *LV = pl.a.data();
 
-    // partitions (matrix_partitioning.c:18-33): equal blocks, remainder last
+    // partitions (matrix_partitioning.c:18-33): equal blocks, remainder last;
+    // or the caller's boundaries (the reference's IS_MATRIX_PARTITIONED path,
+    // matrix_partitioning.c:23, :34-35)
     pl.starts.resize(P + 1);
-    int kb = n / P;
-    for (int t = 0; t < P; t++) pl.starts[t] = kb * t;
-    pl.starts[P] = n;
-    pl.kmax = n - kb * (P - 1);
+    if (opt.starts.empty()) {
+        int kb = n / P;
+        for (int t = 0; t < P; t++) pl.starts[t] = kb * t;
+        pl.starts[P] = n;
+    } else {
+        pl.starts = opt.starts;
+    }
+    pl.kmax = 0;
+    for (int t = 0; t < P; t++) pl.kmax = std::max(pl.kmax, pl.starts[t + 1] - pl.starts[t]);
     if (pl.kmax > opt.kmax_limit)
         return "partition of " + std::to_string(pl.kmax) + " rows exceeds the device limit of " +
                std::to_string(opt.kmax_limit) + " rows; use more partitions";
@@ -273,11 +285,11 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
         pt.s = pl.starts[t];
         pt.k = pl.starts[t + 1] - pt.s;
     }
-    for (int i = 0; i < n; i++) {
+    for (int t = 0; t < P; t++) {
         // coupling entries come first in a row (ascending columns)
-        int t = std::min(i / std::max(kb, 1), P - 1);
-        int s = pl.starts[t];
-        for (int j = LA[i]; j < LA[i + 1] && LJ[j] < s; j++) isdep[LJ[j]] = 1;
+        const int s = pl.starts[t];
+        for (int i = s; i < pl.starts[t + 1]; i++)
+            for (int j = LA[i]; j < LA[i + 1] && LJ[j] < s; j++) isdep[LJ[j]] = 1;
     }
     pl.ridx.assign(n, -1);
     for (int i = 0; i < n; i++)

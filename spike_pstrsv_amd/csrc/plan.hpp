@@ -91,6 +91,9 @@ struct PlanOptions {
     int lanes_per_row = 0;      // 0 = automatic
     int kmax_limit = 16384;     // largest partition a workgroup can hold in LDS
     bool forward = false;       // use the forwarding layout when G == 16
+    // caller partition boundaries (P+1 values, 0 = starts[0] < ... <
+    // starts[P] = n); empty: the reference's formula (k = n/P, remainder last)
+    std::vector<int> starts;
 };
 
 // Builds the plan.  Returns an empty string on success, else an error message.

@@ -85,6 +85,31 @@ def params(seed: int, stream: int, count: int) -> np.ndarray:
     return out
 
 
+def batch_specs(count: int, n_lo: int = 1000, n_hi: int = 65536, seed: int = SEED_BASE + 4) -> list[dict]:
+    """Per-matrix parameters of a batch of banded matrices: n log-uniform in
+    [n_lo, n_hi], bandwidth uniform in {8..128}, off-diagonals per row uniform
+    in {4..24} (at most the bandwidth), 1-8 right-hand sides; each drawn from
+    the matrix's own counter stream."""
+    out = []
+    lo, hi = np.log(n_lo), np.log(n_hi)
+    for i in range(count):
+        u = params(seed, i, 4)
+        n = int(np.floor(np.exp(lo + (hi - lo) * u[0])))
+        n = min(max(n, n_lo), n_hi)
+        bw = 8 + min(int(u[1] * 121), 120)
+        q = min(4 + min(int(u[2] * 21), 20), bw)
+        nrhs = 1 + min(int(u[3] * 8), 7)
+        out.append({"n": n, "bw": bw, "q": q, "nrhs": nrhs, "seed": seed + 0x1000 * (i + 1)})
+    return out
+
+
+def batched(count: int, n_lo: int = 1000, n_hi: int = 65536, seed: int = SEED_BASE + 4):
+    """(specs, matrices): the banded matrices of batch_specs (2% long-range rows)."""
+    specs = batch_specs(count, n_lo, n_hi, seed)
+    mats = [banded(s["n"], s["bw"], s["q"], 0.02, s["seed"]) for s in specs]
+    return specs, mats
+
+
 def to_upper(m: CSR) -> CSR:
     """Index reversal U = J L J: the upper-triangular system with U (J x) = J b."""
     n = m.n

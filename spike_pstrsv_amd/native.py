@@ -28,6 +28,8 @@ EXPORTED = [
     "spike_factor_destroy",
     "spike_factor_stats", "spike_factor_log", "spike_analyze", "spike_pstrsv_set_partitions",
     "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
+    "spike_batch_options_default", "spike_batch_create", "spike_batch_offset", "spike_batch_length",
+    "spike_batch_solve", "spike_batch_stats", "spike_batch_destroy",
 ]
 
 
@@ -48,6 +50,29 @@ class Stats(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchMatrix(C.Structure):
+    _fields_ = [("n", C.c_int), ("a", C.c_void_p), ("ia", C.c_void_p), ("ja", C.c_void_p), ("nrhs", C.c_int)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("rows_per_partition", C.c_int), ("lanes_per_row", C.c_int), ("precision", C.c_int),
+                ("use_graph", C.c_int), ("autotune_sample", C.c_int), ("drop_tol", C.c_double),
+                ("refine_tol", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("count", C.c_int), ("groups", C.c_int), ("rows", C.c_int64), ("nnz", C.c_int64),
+                ("rows_rhs", C.c_int64), ("partitions", C.c_int64), ("n_reduced", C.c_int64),
+                ("rows_per_partition", C.c_int * 4), ("refine_steps", C.c_int), ("refine_est_err", C.c_double),
+                ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double), ("autotune_seconds", C.c_double),
+                ("graph_nodes", C.c_int)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["rows_per_partition"] = list(self.rows_per_partition)
+        return d
 
 
 class SpikeError(RuntimeError):
@@ -92,6 +117,20 @@ lib.spike_pstrsv_current_factor.restype = _vp
 lib.spike_last_error.argtypes = []
 lib.spike_last_error.restype = C.c_char_p
 lib.spike_pstrsv_get_timers.argtypes = [C.POINTER(_d), _i]
+lib.spike_batch_options_default.argtypes = [C.POINTER(BatchOptions)]
+lib.spike_batch_options_default.restype = None
+lib.spike_batch_create.argtypes = [C.POINTER(_vp), _c, _i, C.POINTER(BatchMatrix), C.POINTER(BatchOptions)]
+lib.spike_batch_create.restype = _i
+lib.spike_batch_offset.argtypes = [_vp, _i]
+lib.spike_batch_offset.restype = C.c_int64
+lib.spike_batch_length.argtypes = [_vp]
+lib.spike_batch_length.restype = C.c_int64
+lib.spike_batch_solve.argtypes = [_vp, _vp, _vp, _vp]
+lib.spike_batch_solve.restype = _i
+lib.spike_batch_stats.argtypes = [_vp, C.POINTER(BatchStats)]
+lib.spike_batch_stats.restype = _i
+lib.spike_batch_destroy.argtypes = [_vp]
+lib.spike_batch_destroy.restype = _i
 lib.spike_pstrsv_get_timers.restype = _i
 
 
@@ -239,3 +278,60 @@ def current_stats() -> dict | None:
     st = Stats()
     _check(lib.spike_factor_stats(h, C.byref(st)))
     return st.as_dict()
+
+
+class Batch:
+    """Many independent matrices preprocessed together (spike_batch_t).
+
+    mats: sequence of (a, ia, ja, nrhs).  Right-hand sides and solutions use
+    the packed layout: matrix i's nrhs columns (leading dimension n_i) start
+    at element offsets[i]."""
+
+    def __init__(self, uplo: str, mats, rows_per_partition: int = 0, lanes_per_row: int = 0,
+                 precision: str = "fp64", use_graph: bool = True, autotune_sample: int = 0,
+                 drop_tol: float = -1.0, refine_tol: float = 0.0):
+        self._keep = []
+        arr = (BatchMatrix * len(mats))()
+        for i, (a, ia, ja, nrhs) in enumerate(mats):
+            a, ia, ja = _csr(a, ia, ja)
+            self._keep.append((a, ia, ja))
+            arr[i] = BatchMatrix(len(ia) - 1, a.ctypes.data, ia.ctypes.data, ja.ctypes.data, int(nrhs))
+        o = BatchOptions()
+        lib.spike_batch_options_default(C.byref(o))
+        o.rows_per_partition = rows_per_partition
+        o.lanes_per_row = lanes_per_row
+        o.precision = PRECISIONS[precision]
+        o.use_graph = int(use_graph)
+        o.autotune_sample = autotune_sample
+        o.drop_tol = drop_tol
+        o.refine_tol = refine_tol
+        h = _vp()
+        _check(lib.spike_batch_create(C.byref(h), uplo.encode()[:1], len(mats), arr, C.byref(o)))
+        self._keep = None          # the batch holds its own copies
+        self.handle = h.value
+        self.count = len(mats)
+        self.length = int(lib.spike_batch_length(self.handle))
+        self.offsets = [int(lib.spike_batch_offset(self.handle, i)) for i in range(self.count + 1)]
+
+    def solve(self, b, x=None, stream: int = 0):
+        """b, x: packed float64 arrays (host) or device addresses."""
+        if x is None:
+            x = np.zeros(self.length)
+        _check(lib.spike_batch_solve(self.handle, _ptr(b), _ptr(x), stream or None))
+        return x
+
+    def stats(self) -> dict:
+        st = BatchStats()
+        _check(lib.spike_batch_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.spike_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
