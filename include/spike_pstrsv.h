@@ -102,8 +102,25 @@ typedef struct {
                               /* solve; < 0 = calibrated at preprocessing on a */
                               /* random right-hand side until the predicted    */
                               /* relative error is <= refine_tol               */
-    double refine_tol;        /* target of the calibration; <= 0 = 1e-14       */
+    double refine_tol;        /* target of the calibration; <= 0 = 1e-13       */
+    int method;               /* SPIKE_METHOD_*                                */
+    int use_graph;            /* 1: a device-pointer solve whose (b, x, ldb,   */
+                              /* ldx, nrhs) repeats the previous call's is     */
+                              /* captured into a hipGraph once and replayed    */
+                              /* (up to 16 cached per factor)                  */
 } spike_options_t;
+
+/* Solve methods. */
+enum {
+    SPIKE_METHOD_AUTO = 0,    /* Spike unless its reduced system degenerates   */
+                              /* (dependents > n/8) or the level schedule is   */
+                              /* cheaper by the cost model                     */
+    SPIKE_METHOD_SPIKE = 1,   /* Spike partitions + reduced system             */
+    SPIKE_METHOD_LEVELS = 2   /* level-scheduled substitution over the whole   */
+                              /* GPU, one launch per dependency level (for     */
+                              /* couplings wider than a partition, e.g. 3D     */
+                              /* grids)                                        */
+};
 
 /* Reduced-system solvers. */
 enum {
@@ -159,6 +176,9 @@ typedef struct {
     int refine_steps;         /* refinement steps per solve (FP32_REFINED)     */
     double refine_rate;       /* calibrated contraction per refinement step    */
     double refine_est_err;    /* calibrated predicted relative error           */
+    int method;               /* SPIKE_METHOD_SPIKE or SPIKE_METHOD_LEVELS     */
+    int levels;               /* dependency levels of the whole matrix (0 if   */
+                              /* not computed)                                 */
 } spike_stats_t;
 
 int spike_factor_stats(spike_factor_t f, spike_stats_t *st);
@@ -234,6 +254,56 @@ int spike_batch_solve(spike_batch_t bt, const double *b, double *x, void *stream
 
 int spike_batch_stats(spike_batch_t bt, spike_batch_stats_t *st);
 int spike_batch_destroy(spike_batch_t bt);
+
+/* ------------------------------------------------------------------ */
+/* Serving: resident factors, batched solve requests                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    size_t hbm_budget;        /* bytes for resident matrices (factor + staging)*/
+                              /* 0 = 80% of the free device memory             */
+    int slots;                /* most solves in flight; 0 = 64                 */
+    int streams;              /* stream pool; 0 = 8                            */
+    double window_us;         /* batching window after the oldest waiting      */
+                              /* request; < 0 = 200                            */
+    int max_batch_rhs;        /* right-hand sides merged into one solve, and   */
+                              /* the most one request may carry; 0 = 8         */
+} spike_server_options_t;
+
+typedef struct {
+    int matrices;
+    int64_t resident_bytes;
+    int64_t completed;        /* requests                                      */
+    int64_t solves;           /* merged solves launched                        */
+    int64_t windows;
+    int64_t merged_rhs;       /* right-hand sides in solves of > 1 column      */
+    int max_inflight;
+    int64_t rows_rhs;         /* sum of n x nrhs over completed requests       */
+    double p50_us, p99_us, max_us;   /* latency submit -> completion           */
+    double seconds;           /* first submit -> last completion               */
+    double rows_rhs_per_s;
+} spike_server_stats_t;
+
+typedef struct spike_server_s *spike_server_t;
+
+void spike_server_options_default(spike_server_options_t *o);
+int spike_server_create(spike_server_t *s, const spike_server_options_t *o);
+/* Preprocesses a matrix and keeps it resident; fails with SPIKE_ERR_NOMEM
+ * when its footprint does not fit the remaining budget.  *id names it. */
+int spike_server_add(spike_server_t s, char uplo, int n, const double *a, const int *ia, const int *ja,
+                     const spike_options_t *opts, int *id);
+/* Queues a solve of matrix id for nrhs right-hand sides (device pointers,
+ * column-major); b and x must stay valid until the ticket completes. */
+int spike_server_submit(spike_server_t s, int id, int nrhs, const double *b, int ldb, double *x, int ldx,
+                        uint64_t *ticket);
+/* Blocks until the ticket's solve has completed (x written). */
+int spike_server_wait(spike_server_t s, uint64_t ticket);
+/* Blocks until every queued request has completed. */
+int spike_server_drain(spike_server_t s);
+int spike_server_stats(spike_server_t s, spike_server_stats_t *st);
+int spike_server_reset_stats(spike_server_t s);
+int spike_server_destroy(spike_server_t s);
+const char *spike_server_last_error(void);
 
 /* Host-only symbolic analysis (no device needed): fills st with the sizes the
  * factorisation would have, except fields that need the spikes (nnz_reduced,

@@ -103,6 +103,19 @@ struct spike_factor_s {
     int batch_slots = 0;        // RHS slots of every launch of a batch factor
     int batch_max_nrhs = 0;
     int64_t vec_len = 0;        // batch: packed length of b and x
+    // SPIKE_METHOD_LEVELS: rows grouped by dependency level (lev_ptr on the
+    // host delimits the launches), the lower-form CSR and 1/diag on the
+    // device, a correction workspace for refinement
+    int method = SPIKE_METHOD_SPIKE;
+    int nlev_full = 0;
+    std::vector<int> lev_ptr;
+    int *d_lrows = nullptr;
+    float *d_a32 = nullptr, *d_invd32 = nullptr;
+    double *d_dlt = nullptr;
+    int lev_G = 1;
+    // row blocks of the residual kernel (the partitions, or 1024-row blocks)
+    int nres = 0;
+    DevPart *d_res = nullptr;
 
     template <class T>
     int dalloc(T **p, size_t count)
@@ -126,8 +139,41 @@ struct spike_factor_s {
         if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
         return SPIKE_OK;
     }
+    // Solve graphs for device-pointer solves: a key (b, x, ldb, ldx, nrhs)
+    // seen on two consecutive calls is captured once and replayed.
+    struct GraphKey {
+        const double *b;
+        double *x;
+        int ldb, ldx, nrhs;
+        bool operator==(const GraphKey &o) const
+        {
+            return b == o.b && x == o.x && ldb == o.ldb && ldx == o.ldx && nrhs == o.nrhs;
+        }
+    };
+    struct GraphEntry {
+        GraphKey key;
+        hipGraphExec_t exec;
+        hipGraph_t graph;
+        uint64_t used;
+    };
+    std::vector<GraphEntry> graphs;
+    GraphKey last_key{nullptr, nullptr, 0, 0, 0};
+    uint64_t graph_clock = 0;
+    hipStream_t cap_stream = nullptr;
+
+    void drop_graphs()
+    {
+        for (GraphEntry &g : graphs) {
+            (void)hipGraphExecDestroy(g.exec);
+            (void)hipGraphDestroy(g.graph);
+        }
+        graphs.clear();
+    }
     void release()
     {
+        drop_graphs();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        cap_stream = nullptr;
         for (void *p : allocs) (void)hipFree(p);
         allocs.clear();
         device_bytes = 0;
@@ -241,6 +287,99 @@ void choose_rhs_slots(spike_factor_s *f)
 
 int calibrate_refinement(spike_factor_s *f);
 
+// Automatic partition count: partitions of about 16x the coupling bandwidth
+// (95th percentile of the distance from a row to its farthest column, so rare
+// long-range entries do not count), at least 1024 rows and at most kmax_limit.
+// Partitions much longer than the bandwidth keep the dependents (and so the
+// reduced system) a small fraction of n.
+int default_partitions(char uplo, int n, const int *ia, const int *ja, int kmax_limit)
+{
+    std::vector<int> reach;
+    const int stride = std::max(1, n / (1 << 16));
+    const bool up = uplo == 'U' || uplo == 'u';
+    for (int i = 0; i < n; i += stride) {
+        if (ia[i + 1] <= ia[i]) continue;
+        reach.push_back(up ? ja[ia[i + 1] - 1] - i : i - ja[ia[i]]);
+    }
+    long bw = 1;
+    if (!reach.empty()) {
+        const size_t q = (reach.size() * 95) / 100;
+        std::nth_element(reach.begin(), reach.begin() + q, reach.end());
+        bw = std::max(1, reach[q]);
+    }
+    const long k = std::min<long>(std::max<long>(1024, 16 * bw), std::max(kmax_limit, 1));
+    return (int)std::min<long>(n, std::max<long>(1, (n + k / 2) / k));
+}
+
+// Row blocks of the residual kernel: 64 rows each, cut from the given
+// partition records (or from one record covering the whole matrix), so that
+// every block keeps its matrix's right-hand-side addressing.
+int upload_row_blocks(spike_factor_s *f, const std::vector<DevPart> &parts)
+{
+    std::vector<DevPart> rb;
+    for (const DevPart &p : parts)
+        for (int s0 = 0; s0 < p.k; s0 += 64) {
+            DevPart d = p;
+            d.s = p.s + s0;
+            d.k = std::min(64, p.k - s0);
+            rb.push_back(d);
+        }
+    f->nres = (int)rb.size();
+    return f->upload(&f->d_res, rb);
+}
+
+// Device data of the level-scheduled method.
+int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
+{
+    Plan &pl = f->plan;
+    const int n = pl.n;
+    int rc;
+    int *d_ia, *d_ja;
+    double *d_a, *d_invd;
+    if ((rc = f->upload(&f->d_lrows, rows)) || (rc = f->upload(&d_ia, pl.ia)) || (rc = f->upload(&d_ja, pl.ja)) ||
+        (rc = f->upload(&d_a, pl.a)) || (rc = f->upload(&d_invd, pl.inv_diag)))
+        return rc;
+    f->d_a = d_a;
+    f->d_ia = d_ia;
+    f->d_ja = d_ja;
+    f->d_inv_diag = d_invd;
+    if (f->fp32 && ((rc = f->upload(&f->d_a32, to_f32(pl.a))) || (rc = f->upload(&f->d_invd32, to_f32(pl.inv_diag)))))
+        return rc;
+    std::vector<int> len(n);
+    for (int i = 0; i < n; i++) len[i] = pl.ia[i + 1] - pl.ia[i] - 1;
+    f->lev_G = f->opts.lanes_per_row > 0 ? std::min(f->opts.lanes_per_row, 16) : spike::choose_lanes(len);
+    f->max_nr = 8;
+    {
+        DevPart whole{};
+        whole.s = 0;
+        whole.k = n;
+        whole.mn = n;
+        whole.mflip = pl.flip ? 1 : 0;
+        if ((rc = upload_row_blocks(f, std::vector<DevPart>{whole}))) return rc;
+    }
+    if (f->fp32 && ((rc = f->dalloc(&f->d_r, (size_t)n * 8)) || (rc = f->dalloc(&f->d_dlt, (size_t)n * 8)))) return rc;
+    pl.a.clear(); pl.a.shrink_to_fit();
+    pl.ja.clear(); pl.ja.shrink_to_fit();
+    pl.steps = spike::StepStream();
+    spike_stats_t &st = f->stats;
+    st.n = n;
+    st.partitions = 0;
+    st.nnz = pl.nnz;
+    st.lanes_per_row = f->lev_G;
+    st.rhs_per_launch = f->max_nr;
+    st.precision = f->opts.precision;
+    st.method = SPIKE_METHOD_LEVELS;
+    st.levels = f->nlev_full;
+    if (f->fp32 && (rc = calibrate_refinement(f))) return rc;
+    st.refine_steps = f->refine_steps;
+    st.refine_rate = f->refine_rate;
+    st.refine_est_err = f->refine_est;
+    st.device_bytes = f->device_bytes;
+    st.preproc_seconds = now_s() - t0;
+    g_timers[0] += st.preproc_seconds;
+    return SPIKE_OK;
+}
+
 int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int *ia, const int *ja,
                  const std::vector<int> *starts = nullptr)
 {
@@ -250,7 +389,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     spike::PlanOptions po;
     po.kmax_limit = kmax_limit_for_device();
     int P = f->opts.partitions;
-    if (P <= 0) P = std::max(1, n / 1024);
+    if (P <= 0) P = default_partitions(uplo, n, ia, ja, po.kmax_limit);
     P = std::min(P, n);
     // raise P until the largest partition fits the LDS x segment
     while (P < n && n - (n / P) * (P - 1) > po.kmax_limit) P++;
@@ -268,6 +407,38 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     Plan &pl = f->plan;
     const double t1 = now_s();
     g_timers[1] += t1 - t0;
+    {
+        // dependency levels of the whole matrix (diagonal last)
+        std::vector<int> lev(n, 0);
+        int nl = 0;
+        for (int i = 0; i < n; i++) {
+            int l = 0;
+            for (int e = pl.ia[i]; e < pl.ia[i + 1] - 1; e++) l = std::max(l, lev[pl.ja[e]] + 1);
+            lev[i] = l;
+            nl = std::max(nl, l + 1);
+        }
+        f->nlev_full = nl;
+        // cost model (microseconds): a level launch ~3 plus streaming the
+        // matrix at ~4 TB/s; Spike ~0.19 per block-solve step and pass plus
+        // ~7 Jacobi sweeps over ~25 entries per dependent
+        const double nd = (double)pl.dep_rows.size();
+        const double t_lev = nl * 3.0 + (double)pl.nnz * 12.0 / 4e6;
+        const double t_spk = 2.0 * pl.steps.max_steps * 0.19 + (nd > 0 ? 7.0 * (2.5 + nd * 25.0 * 12.0 / 4e6) : 0.0);
+        int method = f->opts.method;
+        if (!f->mats.empty()) method = SPIKE_METHOD_SPIKE;
+        if (method == SPIKE_METHOD_AUTO)
+            method = (nd > n / 8.0 || t_lev < t_spk) ? SPIKE_METHOD_LEVELS : SPIKE_METHOD_SPIKE;
+        f->method = method;
+        if (method == SPIKE_METHOD_LEVELS) {
+            std::vector<int> ptr(nl + 1, 0), rows(n);
+            for (int i = 0; i < n; i++) ptr[lev[i] + 1]++;
+            for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+            f->lev_ptr = ptr;
+            return build_levels(f, rows, t0);
+        }
+    }
 
     // ---- per-partition records, dependents, coupling rows
     std::vector<DevPart> hp(P);
@@ -556,7 +727,12 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.jacobi_bound = f->sweeps > 0 ? jbound : 0.0;
     st.rhs_per_launch = f->max_nr;
     st.precision = f->opts.precision;
-    if (f->fp32 && (rc = calibrate_refinement(f))) return rc;
+    st.method = SPIKE_METHOD_SPIKE;
+    st.levels = f->nlev_full;
+    if (f->fp32) {
+        if ((rc = upload_row_blocks(f, hp))) return rc;
+        if ((rc = calibrate_refinement(f))) return rc;
+    }
     st.refine_steps = f->refine_steps;
     st.refine_rate = f->refine_rate;
     st.refine_est_err = f->refine_est;
@@ -618,9 +794,48 @@ spike::BlockArgs make_args(spike_factor_s *f, const double *b, int ldb, double *
 // device: x = M^-1 b, or x += M^-1 b with accum (refinement update).  ev
 // (optional, 5 events) brackets the phases; their device times are added to
 // ph[0..3].
+int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
+                 hipEvent_t *ev, double *ph)
+{
+    const int n = f->plan.n;
+    spike::LevelArgs la{};
+    la.rows = f->d_lrows;
+    la.ia = f->d_ia;
+    la.ja = f->d_ja;
+    la.val = f->d_a;
+    la.invd = f->d_inv_diag;
+    la.val32 = f->d_a32;
+    la.invd32 = f->d_invd32;
+    la.b = b;
+    la.ldb = ldb;
+    la.x = accum ? f->d_dlt : x;
+    la.ldx = accum ? n : ldx;
+    la.nr = nr;
+    int slots = 1;
+    while (slots < nr) slots <<= 1;
+    la.nr_slots = slots;
+    la.n = n;
+    la.flip = f->plan.flip ? 1 : 0;
+    la.G = f->lev_G;
+    la.fp32 = f->fp32 ? 1 : 0;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+    for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++)
+        HIP_TRY(spike::launch_level(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], la, s));
+    if (accum) HIP_TRY(spike::launch_axpy(n, nr, f->d_dlt, n, x, ldx, s));
+    if (ev) {
+        HIP_TRY(hipEventRecord(ev[1], s));
+        HIP_TRY(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        ph[0] += ms;
+    }
+    return SPIKE_OK;
+}
+
 int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
                hipEvent_t *ev, double *ph)
 {
+    if (f->method == SPIKE_METHOD_LEVELS) return solve_levels(f, b, ldb, x, ldx, nr, accum, s, ev, ph);
     const Plan &pl = f->plan;
     spike::BlockArgs ba = make_args(f, b, ldb, x, ldx, nr, accum);
     const int slots = ba.nr_slots;
@@ -673,7 +888,7 @@ int residual(spike_factor_s *f, const double *b, int ldb, const double *x, int l
 {
     spike::BlockArgs ra = make_args(f, b, ldb, const_cast<double *>(x), ldx, nr, 0);
     const int nr_max = f->mats.empty() ? nr : f->batch_max_nrhs;
-    HIP_TRY(spike::launch_residual(f->plan.P, f->d_parts, f->d_ia, f->d_ja, f->d_a, ra, nr_max, f->d_r, f->plan.n, s));
+    HIP_TRY(spike::launch_residual(f->nres, f->d_res, f->d_ia, f->d_ja, f->d_a, ra, nr_max, f->d_r, f->plan.n, s));
     return SPIKE_OK;
 }
 
@@ -778,6 +993,63 @@ int calibrate_refinement(spike_factor_s *f)
     return SPIKE_OK;
 }
 
+// Every chunk of a device-pointer solve, launched on s.
+int solve_all(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, int ldx, hipStream_t s)
+{
+    for (int j = 0; j < nrhs; j += f->max_nr) {
+        const int nr = std::min(f->max_nr, nrhs - j);
+        int rc = solve_chunk(f, b + (size_t)j * ldb, ldb, x + (size_t)j * ldx, ldx, nr, s, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return SPIKE_OK;
+}
+
+// Device-pointer solve through the graph cache.  Returns 1 when no graph is
+// used for this call (the caller launches eagerly).
+int solve_graph(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, int ldx, hipStream_t s)
+{
+    const spike_factor_s::GraphKey key{b, x, ldb, ldx, nrhs};
+    f->graph_clock++;
+    for (auto &g : f->graphs)
+        if (g.key == key) {
+            g.used = f->graph_clock;
+            HIP_TRY(hipGraphLaunch(g.exec, s));
+            return SPIKE_OK;
+        }
+    const bool repeat = key == f->last_key;
+    f->last_key = key;
+    if (!repeat) return 1;
+    if (!f->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&f->cap_stream, hipStreamNonBlocking));
+    // the capture stream must start after work already queued on s
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipStreamBeginCapture(f->cap_stream, hipStreamCaptureModeRelaxed));
+    int rc = solve_all(f, nrhs, b, ldb, x, ldx, f->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(f->cap_stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return rc ? rc : fail(SPIKE_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SPIKE_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    constexpr size_t kMaxGraphs = 16;
+    if (f->graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(f->graphs.begin(), f->graphs.end(),
+                                    [](const auto &a, const auto &c) { return a.used < c.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        (void)hipGraphDestroy(lru->graph);
+        f->graphs.erase(lru);
+    }
+    f->graphs.push_back(spike_factor_s::GraphEntry{key, ex, g, f->graph_clock});
+    HIP_TRY(hipGraphLaunch(ex, s));
+    return SPIKE_OK;
+}
+
 int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, int ldx, hipStream_t s,
                  bool timed, double *phase_ms = nullptr)
 {
@@ -789,6 +1061,10 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
     HIP_TRY(hipGetDevice(&dev));
     if (dev != f->device) HIP_TRY(hipSetDevice(f->device));
     const bool bdev = is_device_ptr(b), xdev = is_device_ptr(x);
+    if (bdev && xdev && !timed && f->opts.use_graph) {
+        int rc = solve_graph(f, nrhs, b, ldb, x, ldx, s);
+        if (rc != 1) return rc;
+    }
     const double *db = b;
     double *dx = x;
     int ldb_d = ldb, ldx_d = ldx;
@@ -1342,6 +1618,8 @@ void spike_options_default(spike_options_t *o)
     o->reduced_solver = SPIKE_REDUCED_AUTO;
     o->refine_steps = -1;
     o->refine_tol = 0.0;
+    o->method = SPIKE_METHOD_AUTO;
+    o->use_graph = 1;
 }
 
 int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, const int *ia, const int *ja,
@@ -1436,7 +1714,8 @@ int spike_analyze(char uplo, int n, const double *a, const int *ia, const int *j
     spike_options_default(&o);
     if (opts) o = *opts;
     spike::PlanOptions po;
-    int P = o.partitions > 0 ? o.partitions : std::max(1, n / 1024);
+    if (!ia || !ja) return fail(SPIKE_ERR_ARG, "ia and ja must be non-NULL");
+    int P = o.partitions > 0 ? o.partitions : default_partitions(uplo, n, ia, ja, kmax_limit_for_device());
     po.P = std::min(P, n);
     po.lanes_per_row = o.lanes_per_row;
     Plan pl;

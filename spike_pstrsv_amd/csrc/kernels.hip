@@ -683,6 +683,57 @@ __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ pa
     }
 }
 
+// ---------------------------------------------------------------- levels
+
+// One dependency level of a lower-form triangular solve: rows
+// rows[r0 .. r0+cnt) depend only on rows of earlier levels, so G lanes per
+// row compute x[row] = (b[row] - sum_j a_rj x[j]) / a_rr in parallel, reading
+// earlier x from global memory (the previous launch has completed).  Values
+// and accumulation in T (FP64, or FP32 for the refined mode); b and x are
+// FP64 with the caller's index order (flip: n-1-i).
+template <int G, int NR, class T>
+__global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int *__restrict__ rows,
+                                               const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const T *__restrict__ val, const T *__restrict__ invd,
+                                               const double *__restrict__ b, int ldb, double *x, int ldx,
+                                               int nr, int n, int flip)
+{
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int q = gid / G, li = gid % G;
+    const bool ok = q < cnt;
+    const int row = ok ? rows[r0 + q] : 0;
+    T acc[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = (T)0;
+    if (ok) {
+        const int e1 = ia[row + 1] - 1;   // the diagonal is last
+        for (int e = ia[row] + li; e < e1; e += G) {
+            const T v = val[e];
+            const int c = ja[e], cm = flip ? n - 1 - c : c;
+#pragma unroll
+            for (int t = 0; t < NR; t++)
+                if (t < nr) acc[t] = fma(v, (T)x[(size_t)t * ldx + cm], acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
+    if (ok && li == 0) {
+        const int gm = flip ? n - 1 - row : row;
+        const T d = invd[row];
+#pragma unroll
+        for (int t = 0; t < NR; t++)
+            if (t < nr) x[(size_t)t * ldx + gm] = (double)(((T)b[(size_t)t * ldb + gm] - acc[t]) * d);
+    }
+}
+
+// x[:, t] += d[:, t] for t < nr (refinement update of the level method).
+__global__ __launch_bounds__(256) void k_axpy(int n, int nr, const double *__restrict__ d, int ldd, double *x, int ldx)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    for (int t = 0; t < nr; t++) x[(size_t)t * ldx + i] += d[(size_t)t * ldd + i];
+}
+
 // ---------------------------------------------------------------- spikes
 
 __global__ __launch_bounds__(256) void k_spike_build(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
@@ -878,6 +929,55 @@ hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+template <int G, int NR, class T>
+hipError_t launch_level_t(int r0, int cnt, const LevelArgs &a, hipStream_t s)
+{
+    const dim3 grid((unsigned)(((size_t)cnt * G + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_level<G, NR, T>), grid, block, 0, s, r0, cnt, a.rows, a.ia, a.ja,
+                       (const T *)(a.fp32 ? (const void *)a.val32 : (const void *)a.val),
+                       (const T *)(a.fp32 ? (const void *)a.invd32 : (const void *)a.invd), a.b, a.ldb, a.x, a.ldx,
+                       a.nr, a.n, a.flip);
+    return hipGetLastError();
+}
+
+template <int G, int NR>
+hipError_t launch_level_g(int r0, int cnt, const LevelArgs &a, hipStream_t s)
+{
+    return a.fp32 ? launch_level_t<G, NR, float>(r0, cnt, a, s) : launch_level_t<G, NR, double>(r0, cnt, a, s);
+}
+
+template <int G>
+hipError_t launch_level_nr(int r0, int cnt, const LevelArgs &a, hipStream_t s)
+{
+    switch (a.nr_slots) {
+    case 1: return launch_level_g<G, 1>(r0, cnt, a, s);
+    case 2: return launch_level_g<G, 2>(r0, cnt, a, s);
+    case 4: return launch_level_g<G, 4>(r0, cnt, a, s);
+    case 8: return launch_level_g<G, 8>(r0, cnt, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_level(int r0, int cnt, const LevelArgs &a, hipStream_t s)
+{
+    if (cnt <= 0) return hipSuccess;
+    switch (a.G) {
+    case 1: return launch_level_nr<1>(r0, cnt, a, s);
+    case 2: return launch_level_nr<2>(r0, cnt, a, s);
+    case 4: return launch_level_nr<4>(r0, cnt, a, s);
+    case 8: return launch_level_nr<8>(r0, cnt, a, s);
+    case 16: return launch_level_nr<16>(r0, cnt, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int ldx, hipStream_t s)
+{
+    if (n <= 0 || nr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, n, nr, d, ldd, x, ldx);
+    return hipGetLastError();
 }
 
 hipError_t launch_spike_build(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,

@@ -122,6 +122,23 @@ hipError_t launch_reduced_fill(int nred, const int64_t *wrow, const int *kbeg, c
 hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int *ja, const double *a_,
                            const BlockArgs &a, int nr_max, double *r, int ldr, hipStream_t s);
 
+// Level-scheduled substitution over the lower-form CSR (diagonal last).
+struct LevelArgs {
+    const int *rows;        // rows grouped by dependency level
+    const int *ia, *ja;
+    const double *val;      // FP64 values / 1/diag
+    const double *invd;
+    const float *val32;     // FP32 values / 1/diag (fp32)
+    const float *invd32;
+    const double *b;
+    double *x;
+    int ldb, ldx, nr, nr_slots, n, flip, G, fp32;
+};
+// Rows rows[r0 .. r0+cnt) (one level): x = (b - strict-lower part . x) / diag.
+hipError_t launch_level(int r0, int cnt, const LevelArgs &a, hipStream_t s);
+// x[:, t] += d[:, t], t < nr.
+hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int ldx, hipStream_t s);
+
 // Largest dynamic LDS a block-solve workgroup may use, in bytes.
 size_t block_solve_lds_limit();
 

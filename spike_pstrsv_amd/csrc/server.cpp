@@ -1,0 +1,484 @@
+// Serving loop over resident factors (extension API, include/spike_pstrsv.h).
+//
+// Factors stay preprocessed in device memory; a matrix is admitted while the
+// resident footprint (factor + its staging buffers) fits the HBM budget.
+// Solve requests (device pointers) are queued by spike_server_submit.  A
+// dispatcher thread closes a batching window window_us after the oldest
+// waiting request arrived, merges the window's requests of each matrix into
+// solves of up to max_batch_rhs right-hand sides, and launches them on a pool
+// of streams (matrix id modulo the pool size, so the solves of one factor
+// stay ordered and never share its workspaces).  Right-hand sides are
+// gathered into the matrix's fixed staging buffers and solutions scattered
+// back, so every solve of a matrix has the same pointers and replays the
+// factor's captured hipGraph.  At most `slots` solves are in flight; requests
+// that arrive meanwhile wait and are admitted as solves complete (continuous
+// batching).  Completion is observed by polling events; latency = completion
+// time - submit time.
+#include "spike_pstrsv.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+struct Request {
+    uint64_t ticket;
+    int id, nrhs, ldb, ldx;
+    const double *b;
+    double *x;
+    clk::time_point t_submit;
+};
+
+struct InFlight {
+    hipEvent_t done;
+    std::vector<Request> reqs;
+    int64_t rows_rhs;
+};
+
+struct Job {
+    int id, cols;
+    std::vector<Request> reqs;
+    hipEvent_t ev;
+};
+
+}  // namespace
+
+struct spike_server_s {
+    spike_server_options_t opts{};
+    int device = 0;
+    std::mutex mu;
+    std::condition_variable cv_submit, cv_done;
+    std::vector<spike_factor_t> mats;
+    std::vector<int> mat_n;
+    std::vector<int64_t> mat_bytes;
+    std::vector<double *> stage;          // per matrix: b then x, n x max_batch_rhs each
+    int64_t resident = 0;
+    std::deque<Request> pending;
+    std::vector<InFlight> inflight;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> spare_events;
+    uint64_t next_ticket = 1;
+    std::map<uint64_t, int> finished;     // ticket -> status (until collected)
+    std::vector<double> lat;              // seconds, per completed request
+    int64_t rows_rhs_done = 0;
+    int64_t solves = 0, windows = 0, merged_rhs = 0;
+    int max_inflight = 0;
+    clk::time_point t_first, t_last;
+    bool have_first = false;
+    int inflight_reserved = 0;
+    int error = SPIKE_OK;
+    std::string error_msg;
+    std::atomic<bool> stop{false};
+    std::thread worker;
+
+    void run();
+    void dispatch_window(std::unique_lock<std::mutex> &lk);
+    void poll_done(std::unique_lock<std::mutex> &lk);
+    int launch(const Job &j);
+};
+
+namespace {
+
+thread_local std::string s_err;
+
+int sfail(int code, const std::string &m)
+{
+    s_err = m;
+    return code;
+}
+
+}  // namespace
+
+// Polls in-flight solves and retires the finished ones (mu held).
+void spike_server_s::poll_done(std::unique_lock<std::mutex> &lk)
+{
+    (void)lk;
+    for (size_t i = 0; i < inflight.size();) {
+        hipError_t q = hipEventQuery(inflight[i].done);
+        if (q == hipErrorNotReady) {
+            i++;
+            continue;
+        }
+        const clk::time_point now = clk::now();
+        const int st = q == hipSuccess ? SPIKE_OK : SPIKE_ERR_DEVICE;
+        for (const Request &r : inflight[i].reqs) {
+            finished[r.ticket] = st;
+            lat.push_back(secs(r.t_submit, now));
+        }
+        rows_rhs_done += inflight[i].rows_rhs;
+        t_last = now;
+        spare_events.push_back(inflight[i].done);
+        inflight[i] = std::move(inflight.back());
+        inflight.pop_back();
+        cv_done.notify_all();
+    }
+}
+
+// Launches one merged solve (mu not held): gather, solve, scatter, event.
+int spike_server_s::launch(const Job &j)
+{
+    const int n = mat_n[j.id];
+    hipStream_t s = streams[j.id % streams.size()];
+    double *mb = stage[j.id], *mx = stage[j.id] + (size_t)n * opts.max_batch_rhs;
+    int c = 0;
+    for (const Request &r : j.reqs) {
+        if (hipMemcpy2DAsync(mb + (size_t)c * n, sizeof(double) * n, r.b, sizeof(double) * r.ldb, sizeof(double) * n,
+                             r.nrhs, hipMemcpyDefault, s) != hipSuccess)
+            return SPIKE_ERR_DEVICE;
+        c += r.nrhs;
+    }
+    // solve a power-of-two column count (the kernels' slot count): columns
+    // past j.cols hold earlier right-hand sides and their results are unused
+    int padded = 1;
+    while (padded < j.cols) padded <<= 1;
+    padded = std::min(padded, opts.max_batch_rhs);
+    int rc = spike_factor_solve(mats[j.id], padded, mb, n, mx, n, s);
+    if (rc != SPIKE_OK) return rc;
+    c = 0;
+    for (const Request &r : j.reqs) {
+        if (hipMemcpy2DAsync(r.x, sizeof(double) * r.ldx, mx + (size_t)c * n, sizeof(double) * n, sizeof(double) * n,
+                             r.nrhs, hipMemcpyDefault, s) != hipSuccess)
+            return SPIKE_ERR_DEVICE;
+        c += r.nrhs;
+    }
+    return hipEventRecord(j.ev, s) == hipSuccess ? SPIKE_OK : SPIKE_ERR_DEVICE;
+}
+
+// Dispatches the requests of the oldest window (mu held on entry and exit;
+// released while launching).  Solves beyond the free slots stay pending.
+void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
+{
+    windows++;
+    const clk::time_point w_end = pending.front().t_submit + std::chrono::nanoseconds((int64_t)(opts.window_us * 1e3));
+    std::map<int, std::vector<Request>> by_mat;
+    std::deque<Request> rest;
+    while (!pending.empty()) {
+        Request r = pending.front();
+        pending.pop_front();
+        if (r.t_submit <= w_end) by_mat[r.id].push_back(r);
+        else rest.push_back(r);
+    }
+    std::vector<Job> jobs;
+    std::vector<Request> keep;
+    int free_slots = opts.slots - (int)inflight.size();
+    for (auto &kv : by_mat) {
+        std::vector<Request> &rs = kv.second;
+        size_t i = 0;
+        while (i < rs.size()) {
+            if (free_slots <= 0) {
+                keep.insert(keep.end(), rs.begin() + i, rs.end());
+                break;
+            }
+            Job j;
+            j.id = kv.first;
+            j.cols = 0;
+            while (i < rs.size() && j.cols + rs[i].nrhs <= opts.max_batch_rhs) {
+                j.cols += rs[i].nrhs;
+                j.reqs.push_back(rs[i++]);
+            }
+            if (!spare_events.empty()) {
+                j.ev = spare_events.back();
+                spare_events.pop_back();
+            } else if (hipEventCreateWithFlags(&j.ev, hipEventDisableTiming) != hipSuccess) {
+                keep.insert(keep.end(), j.reqs.begin(), j.reqs.end());
+                keep.insert(keep.end(), rs.begin() + i, rs.end());
+                break;
+            }
+            jobs.push_back(std::move(j));
+            free_slots--;
+        }
+    }
+    // waiting requests go back to the front, in arrival order
+    std::vector<Request> back(keep.begin(), keep.end());
+    back.insert(back.end(), rest.begin(), rest.end());
+    std::sort(back.begin(), back.end(), [](const Request &a, const Request &b) { return a.t_submit < b.t_submit; });
+    for (auto it = back.rbegin(); it != back.rend(); ++it) pending.push_front(*it);
+    if (jobs.empty()) return;
+    // reserve the slots, then launch without the lock
+    std::vector<int> rcs(jobs.size(), SPIKE_OK);
+    inflight_reserved += (int)jobs.size();
+    lk.unlock();
+    for (size_t k = 0; k < jobs.size(); k++) rcs[k] = launch(jobs[k]);
+    lk.lock();
+    inflight_reserved -= (int)jobs.size();
+    for (size_t k = 0; k < jobs.size(); k++) {
+        Job &j = jobs[k];
+        if (rcs[k] != SPIKE_OK) {
+            error = rcs[k];
+            error_msg = spike_last_error();
+            (void)hipStreamSynchronize(streams[j.id % streams.size()]);
+            for (const Request &r : j.reqs) finished[r.ticket] = rcs[k];
+            spare_events.push_back(j.ev);
+            cv_done.notify_all();
+            continue;
+        }
+        InFlight fl;
+        fl.done = j.ev;
+        fl.reqs = std::move(j.reqs);
+        fl.rows_rhs = (int64_t)mat_n[j.id] * j.cols;
+        inflight.push_back(std::move(fl));
+        solves++;
+        if (j.cols > 1) merged_rhs += j.cols;
+    }
+    max_inflight = std::max(max_inflight, (int)inflight.size());
+}
+
+void spike_server_s::run()
+{
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (!stop.load()) {
+        poll_done(lk);
+        if (pending.empty()) {
+            if (inflight.empty())
+                cv_submit.wait_for(lk, std::chrono::milliseconds(5));
+            else
+                cv_submit.wait_for(lk, std::chrono::microseconds(20));
+            continue;
+        }
+        const clk::time_point close =
+            pending.front().t_submit + std::chrono::nanoseconds((int64_t)(opts.window_us * 1e3));
+        const clk::time_point now = clk::now();
+        if (now < close) {
+            // keep retiring solves while the window fills
+            const auto wait = std::min<clk::duration>(close - now, std::chrono::microseconds(20));
+            cv_submit.wait_for(lk, wait);
+            continue;
+        }
+        if ((int)inflight.size() >= opts.slots) {
+            // no free slot: wait for a completion
+            cv_submit.wait_for(lk, std::chrono::microseconds(20));
+            continue;
+        }
+        dispatch_window(lk);
+    }
+    // drain
+    while (!inflight.empty()) {
+        poll_done(lk);
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+    }
+}
+
+extern "C" {
+
+void spike_server_options_default(spike_server_options_t *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->hbm_budget = 0;
+    o->slots = 64;
+    o->streams = 8;
+    o->window_us = 200.0;
+    o->max_batch_rhs = 8;
+}
+
+int spike_server_create(spike_server_t *out, const spike_server_options_t *opts)
+{
+    if (!out) return sfail(SPIKE_ERR_ARG, "server pointer is NULL");
+    *out = nullptr;
+    spike_server_options_t o;
+    spike_server_options_default(&o);
+    if (opts) o = *opts;
+    if (o.streams <= 0) o.streams = 8;
+    if (o.window_us < 0) o.window_us = 200.0;
+    if (o.max_batch_rhs <= 0) o.max_batch_rhs = 8;
+    if (o.slots <= 0) o.slots = 64;
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return sfail(SPIKE_ERR_DEVICE, "hipMemGetInfo failed");
+    if (o.hbm_budget == 0) o.hbm_budget = (size_t)(freeb * 0.8);
+    spike_server_s *s = new spike_server_s();
+    s->opts = o;
+    (void)hipGetDevice(&s->device);
+    for (int i = 0; i < o.streams; i++) {
+        hipStream_t st;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            for (hipStream_t t : s->streams) (void)hipStreamDestroy(t);
+            delete s;
+            return sfail(SPIKE_ERR_DEVICE, "hipStreamCreate failed");
+        }
+        s->streams.push_back(st);
+    }
+    s->worker = std::thread([s]() { s->run(); });
+    *out = s;
+    return SPIKE_OK;
+}
+
+int spike_server_add(spike_server_t s, char uplo, int n, const double *a, const int *ia, const int *ja,
+                     const spike_options_t *fo, int *id)
+{
+    if (!s || !id) return sfail(SPIKE_ERR_ARG, "NULL argument");
+    *id = -1;
+    spike_factor_t f = nullptr;
+    int rc = spike_factor_create(&f, uplo, n, a, ia, ja, fo);
+    if (rc != SPIKE_OK) return sfail(rc, spike_last_error());
+    spike_stats_t st;
+    spike_factor_stats(f, &st);
+    const int64_t stage_bytes = (int64_t)n * s->opts.max_batch_rhs * 2 * (int64_t)sizeof(double);
+    const int64_t need = st.device_bytes + stage_bytes;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->resident + need > (int64_t)s->opts.hbm_budget) {
+        spike_factor_destroy(f);
+        return sfail(SPIKE_ERR_NOMEM, "matrix footprint of " + std::to_string(need) + " bytes exceeds the HBM budget (" +
+                                          std::to_string((int64_t)s->opts.hbm_budget - s->resident) + " bytes left)");
+    }
+    double *stg = nullptr;
+    if (hipMalloc((void **)&stg, (size_t)stage_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        spike_factor_destroy(f);
+        return sfail(SPIKE_ERR_NOMEM, "staging allocation failed");
+    }
+    // zeroed staging; then every padded column count is solved twice so the
+    // factor captures its solve graphs now rather than while serving
+    if (hipMemset(stg, 0, (size_t)stage_bytes) != hipSuccess) {
+        (void)hipFree(stg);
+        spike_factor_destroy(f);
+        return sfail(SPIKE_ERR_DEVICE, "staging clear failed");
+    }
+    {
+        hipStream_t ws = s->streams[s->mats.size() % s->streams.size()];
+        double *mb = stg, *mx = stg + (size_t)n * s->opts.max_batch_rhs;
+        for (int c = 1; c <= s->opts.max_batch_rhs && rc == SPIKE_OK; c <<= 1)
+            for (int rep = 0; rep < 2 && rc == SPIKE_OK; rep++) rc = spike_factor_solve(f, c, mb, n, mx, n, ws);
+        if (rc == SPIKE_OK && hipStreamSynchronize(ws) != hipSuccess) rc = SPIKE_ERR_DEVICE;
+        if (rc != SPIKE_OK) {
+            (void)hipFree(stg);
+            spike_factor_destroy(f);
+            return sfail(rc, "warm-up solve failed");
+        }
+    }
+    s->resident += need;
+    s->mats.push_back(f);
+    s->mat_n.push_back(n);
+    s->mat_bytes.push_back(need);
+    s->stage.push_back(stg);
+    *id = (int)s->mats.size() - 1;
+    return SPIKE_OK;
+}
+
+int spike_server_submit(spike_server_t s, int id, int nrhs, const double *b, int ldb, double *x, int ldx,
+                        uint64_t *ticket)
+{
+    if (!s || !ticket || !b || !x) return sfail(SPIKE_ERR_ARG, "NULL argument");
+    Request r;
+    r.t_submit = clk::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int)s->mats.size()) return sfail(SPIKE_ERR_ARG, "unknown matrix id");
+    if (nrhs < 1 || nrhs > s->opts.max_batch_rhs || ldb < s->mat_n[id] || ldx < s->mat_n[id])
+        return sfail(SPIKE_ERR_ARG, "nrhs must be in [1, max_batch_rhs] and ldb, ldx >= n");
+    if (s->error) return sfail(s->error, "server failed: " + s->error_msg);
+    r.ticket = s->next_ticket++;
+    r.id = id;
+    r.nrhs = nrhs;
+    r.b = b;
+    r.ldb = ldb;
+    r.x = x;
+    r.ldx = ldx;
+    if (!s->have_first) {
+        s->t_first = r.t_submit;
+        s->have_first = true;
+    }
+    s->pending.push_back(r);
+    *ticket = r.ticket;
+    s->cv_submit.notify_one();
+    return SPIKE_OK;
+}
+
+int spike_server_wait(spike_server_t s, uint64_t ticket)
+{
+    if (!s) return sfail(SPIKE_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (ticket == 0 || ticket >= s->next_ticket) return sfail(SPIKE_ERR_ARG, "unknown ticket");
+    for (;;) {
+        auto it = s->finished.find(ticket);
+        if (it != s->finished.end()) {
+            const int st = it->second;
+            s->finished.erase(it);
+            return st == SPIKE_OK ? SPIKE_OK : sfail(st, s->error_msg);
+        }
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+    }
+}
+
+int spike_server_drain(spike_server_t s)
+{
+    if (!s) return sfail(SPIKE_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->pending.empty() || !s->inflight.empty() || s->inflight_reserved > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+    s->finished.clear();
+    return s->error ? sfail(s->error, s->error_msg) : SPIKE_OK;
+}
+
+int spike_server_stats(spike_server_t s, spike_server_stats_t *st)
+{
+    if (!s || !st) return sfail(SPIKE_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(st, 0, sizeof *st);
+    st->matrices = (int)s->mats.size();
+    st->resident_bytes = s->resident;
+    st->completed = (int64_t)s->lat.size();
+    st->solves = s->solves;
+    st->windows = s->windows;
+    st->merged_rhs = s->merged_rhs;
+    st->max_inflight = s->max_inflight;
+    st->rows_rhs = s->rows_rhs_done;
+    if (!s->lat.empty()) {
+        std::vector<double> v(s->lat);
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) { return v[std::min(v.size() - 1, (size_t)(p * (double)v.size()))]; };
+        st->p50_us = pct(0.50) * 1e6;
+        st->p99_us = pct(0.99) * 1e6;
+        st->max_us = v.back() * 1e6;
+        const double span = s->have_first ? secs(s->t_first, s->t_last) : 0.0;
+        st->seconds = span;
+        st->rows_rhs_per_s = span > 0 ? (double)s->rows_rhs_done / span : 0.0;
+    }
+    return SPIKE_OK;
+}
+
+int spike_server_reset_stats(spike_server_t s)
+{
+    if (!s) return sfail(SPIKE_ERR_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat.clear();
+    s->rows_rhs_done = 0;
+    s->solves = s->windows = s->merged_rhs = 0;
+    s->max_inflight = 0;
+    s->have_first = false;
+    return SPIKE_OK;
+}
+
+int spike_server_destroy(spike_server_t s)
+{
+    if (!s) return SPIKE_OK;
+    s->stop.store(true);
+    s->cv_submit.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (hipStream_t t : s->streams) (void)hipStreamDestroy(t);
+    for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
+    for (spike_factor_t f : s->mats) spike_factor_destroy(f);
+    for (double *p : s->stage) (void)hipFree(p);
+    delete s;
+    return SPIKE_OK;
+}
+
+const char *spike_server_last_error(void) { return s_err.c_str(); }
+
+}  // extern "C"

@@ -110,6 +110,36 @@ def batched(count: int, n_lo: int = 1000, n_hi: int = 65536, seed: int = SEED_BA
     return specs, mats
 
 
+def serving_specs(count: int = 64, n_lo: int = 10_000, n_hi: int = 2_000_000, seed: int = SEED_BASE + 5) -> list[dict]:
+    """Resident matrices of the serving workload: n log-uniform in [n_lo, n_hi];
+    even ids are Laplacian lower triangles (alternately 2D with side
+    round(sqrt(n)) and 3D with side round(cbrt(n))), odd ids random banded
+    matrices with long-range rows as in the 1M-row banded workload."""
+    out = []
+    lo, hi = np.log(n_lo), np.log(n_hi)
+    for i in range(count):
+        u = params(seed, i, 1)
+        n = int(np.floor(np.exp(lo + (hi - lo) * u[0])))
+        if i % 2 == 0:
+            if (i // 2) % 2 == 0:
+                side = max(2, int(round(n ** 0.5)))
+                out.append({"kind": "lap2d", "side": side, "n": side * side})
+            else:
+                side = max(2, int(round(n ** (1.0 / 3.0))))
+                out.append({"kind": "lap3d", "side": side, "n": side ** 3})
+        else:
+            out.append({"kind": "banded", "n": n, "seed": seed + 0x1000 * (i + 1)})
+    return out
+
+
+def serving_matrix(spec: dict) -> CSR:
+    if spec["kind"] == "lap2d":
+        return lap2d(spec["side"])
+    if spec["kind"] == "lap3d":
+        return lap3d(spec["side"])
+    return banded(spec["n"], 64, 16, 0.02, spec["seed"])
+
+
 def to_upper(m: CSR) -> CSR:
     """Index reversal U = J L J: the upper-triangular system with U (J x) = J b."""
     n = m.n

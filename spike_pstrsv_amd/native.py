@@ -30,13 +30,17 @@ EXPORTED = [
     "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
     "spike_batch_options_default", "spike_batch_create", "spike_batch_offset", "spike_batch_length",
     "spike_batch_solve", "spike_batch_stats", "spike_batch_destroy",
+    "spike_server_options_default", "spike_server_create", "spike_server_add", "spike_server_submit",
+    "spike_server_wait", "spike_server_drain", "spike_server_stats", "spike_server_reset_stats",
+    "spike_server_destroy", "spike_server_last_error",
 ]
 
 
 class Options(C.Structure):
     _fields_ = [("partitions", C.c_int), ("drop_tol", C.c_double), ("precision", C.c_int),
                 ("lanes_per_row", C.c_int), ("verbose", C.c_int), ("reduced_solver", C.c_int),
-                ("refine_steps", C.c_int), ("refine_tol", C.c_double)]
+                ("refine_steps", C.c_int), ("refine_tol", C.c_double), ("method", C.c_int),
+                ("use_graph", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -46,7 +50,8 @@ class Stats(C.Structure):
                 ("lanes_per_row", C.c_int), ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double),
                 ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double),
                 ("rhs_per_launch", C.c_int), ("precision", C.c_int), ("refine_steps", C.c_int),
-                ("refine_rate", C.c_double), ("refine_est_err", C.c_double)]
+                ("refine_rate", C.c_double), ("refine_est_err", C.c_double), ("method", C.c_int),
+                ("levels", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +78,21 @@ class BatchStats(C.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["rows_per_partition"] = list(self.rows_per_partition)
         return d
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("slots", C.c_int), ("streams", C.c_int), ("window_us", C.c_double),
+                ("max_batch_rhs", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("matrices", C.c_int), ("resident_bytes", C.c_int64), ("completed", C.c_int64),
+                ("solves", C.c_int64), ("windows", C.c_int64), ("merged_rhs", C.c_int64),
+                ("max_inflight", C.c_int), ("rows_rhs", C.c_int64), ("p50_us", C.c_double), ("p99_us", C.c_double),
+                ("max_us", C.c_double), ("seconds", C.c_double), ("rows_rhs_per_s", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SpikeError(RuntimeError):
@@ -131,6 +151,26 @@ lib.spike_batch_stats.argtypes = [_vp, C.POINTER(BatchStats)]
 lib.spike_batch_stats.restype = _i
 lib.spike_batch_destroy.argtypes = [_vp]
 lib.spike_batch_destroy.restype = _i
+lib.spike_server_options_default.argtypes = [C.POINTER(ServerOptions)]
+lib.spike_server_options_default.restype = None
+lib.spike_server_create.argtypes = [C.POINTER(_vp), C.POINTER(ServerOptions)]
+lib.spike_server_create.restype = _i
+lib.spike_server_add.argtypes = [_vp, _c, _i, _vp, _vp, _vp, C.POINTER(Options), C.POINTER(_i)]
+lib.spike_server_add.restype = _i
+lib.spike_server_submit.argtypes = [_vp, _i, _i, _vp, _i, _vp, _i, C.POINTER(C.c_uint64)]
+lib.spike_server_submit.restype = _i
+lib.spike_server_wait.argtypes = [_vp, C.c_uint64]
+lib.spike_server_wait.restype = _i
+lib.spike_server_drain.argtypes = [_vp]
+lib.spike_server_drain.restype = _i
+lib.spike_server_stats.argtypes = [_vp, C.POINTER(ServerStats)]
+lib.spike_server_stats.restype = _i
+lib.spike_server_reset_stats.argtypes = [_vp]
+lib.spike_server_reset_stats.restype = _i
+lib.spike_server_destroy.argtypes = [_vp]
+lib.spike_server_destroy.restype = _i
+lib.spike_server_last_error.argtypes = []
+lib.spike_server_last_error.restype = C.c_char_p
 lib.spike_pstrsv_get_timers.restype = _i
 
 
@@ -163,11 +203,12 @@ def _csr(a, ia, ja):
 
 REDUCED_SOLVERS = {"auto": 0, "exact": 1, "jacobi": 2}
 PRECISIONS = {"fp64": 0, "fp32_refined": 1}
+METHODS = {"auto": 0, "spike": 1, "levels": 2}
 
 
 def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False,
             reduced_solver: str = "auto", precision: str = "fp64", refine_steps: int = -1,
-            refine_tol: float = 0.0) -> Options:
+            refine_tol: float = 0.0, method: str = "auto") -> Options:
     o = Options()
     lib.spike_options_default(C.byref(o))
     o.partitions = partitions
@@ -178,6 +219,7 @@ def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0,
     o.precision = PRECISIONS[precision]
     o.refine_steps = refine_steps
     o.refine_tol = refine_tol
+    o.method = METHODS[method]
     return o
 
 
@@ -186,12 +228,13 @@ class Factor:
 
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
                  lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto",
-                 precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0):
+                 precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0,
+                 method: str = "auto"):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()
         o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver, precision, refine_steps,
-                    refine_tol)
+                    refine_tol, method)
         _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
                                        ja.ctypes.data, C.byref(o)))
         self.handle = h.value
@@ -328,6 +371,72 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib.spike_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _scheck(rc: int) -> None:
+    if rc != SPIKE_OK:
+        raise SpikeError(rc, (lib.spike_server_last_error() or b"").decode())
+
+
+class Server:
+    """Resident factors plus a batching dispatcher (spike_server_t).
+
+    Requests carry device pointers (int addresses or tensors with data_ptr())."""
+
+    def __init__(self, hbm_budget: int = 0, slots: int = 64, streams: int = 8, window_us: float = 200.0,
+                 max_batch_rhs: int = 8):
+        o = ServerOptions()
+        lib.spike_server_options_default(C.byref(o))
+        o.hbm_budget = hbm_budget
+        o.slots = slots
+        o.streams = streams
+        o.window_us = window_us
+        o.max_batch_rhs = max_batch_rhs
+        h = _vp()
+        _scheck(lib.spike_server_create(C.byref(h), C.byref(o)))
+        self.handle = h.value
+        self.n = []
+
+    def add(self, uplo: str, a, ia, ja, **kw) -> int:
+        a, ia, ja = _csr(a, ia, ja)
+        o = options(**kw)
+        mid = C.c_int(-1)
+        _scheck(lib.spike_server_add(self.handle, uplo.encode()[:1], len(ia) - 1, a.ctypes.data, ia.ctypes.data,
+                                     ja.ctypes.data, C.byref(o), C.byref(mid)))
+        self.n.append(len(ia) - 1)
+        return mid.value
+
+    def submit(self, mid: int, nrhs: int, b, x, ldb: int | None = None, ldx: int | None = None) -> int:
+        n = self.n[mid]
+        t = C.c_uint64(0)
+        _scheck(lib.spike_server_submit(self.handle, mid, nrhs, _ptr(b), n if ldb is None else ldb, _ptr(x),
+                                        n if ldx is None else ldx, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> None:
+        _scheck(lib.spike_server_wait(self.handle, ticket))
+
+    def drain(self) -> None:
+        _scheck(lib.spike_server_drain(self.handle))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _scheck(lib.spike_server_stats(self.handle, C.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        _scheck(lib.spike_server_reset_stats(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.spike_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

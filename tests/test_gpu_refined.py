@@ -104,3 +104,45 @@ def test_refined_device_pointers_and_profiled(native):
     ph = f.solve_profiled(db.data_ptr(), dx.data_ptr(), nrhs=3, stream=s)
     assert len(ph) == 4 and all(v > 0 for v in ph)
     assert np.array_equal(dx.cpu().numpy(), X)
+
+
+# ---------------------------------------------------------------- level method
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_levels_chosen_for_lap3d(native, uplo):
+    """Plane coupling wider than a partition: the cost model picks the level
+    schedule (the Spike reduced system would hold nearly every row)."""
+    m = gen.lap3d(64)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    B = gen.rhs(m.n, 3, seed=gen.SEED_BASE + 150)
+    f = native.Factor(uplo, m.a, m.ia, m.ja)
+    st = f.stats()
+    print("lap3d_64", uplo, st)
+    assert st["method"] == 2 and st["levels"] == 3 * 63 + 1
+    X = f.solve(B, nrhs=3)
+    for j in range(3):
+        assert relerr(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])) <= TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+@pytest.mark.parametrize("nrhs", [1, 5, 11])
+def test_levels_forced_banded(native, precision, nrhs):
+    m = gen.banded(1 << 13, 32, 8, 0.05, gen.SEED_BASE + 160)
+    B = gen.rhs(m.n, nrhs, seed=gen.SEED_BASE + 161).reshape(nrhs, m.n)
+    f = native.Factor("L", m.a, m.ia, m.ja, method="levels", precision=precision)
+    assert f.stats()["method"] == 2
+    X = f.solve(B, nrhs=nrhs).reshape(nrhs, m.n)
+    for j in range(nrhs):
+        assert relerr(X[j], orc.ldsol(m.a, m.ia, m.ja, B[j])) <= TOL
+
+
+def test_levels_lap3d_64_refined_multi_rhs(native):
+    m = gen.lap3d(64)
+    B = gen.rhs(m.n, 4, seed=gen.SEED_BASE + 170)
+    f = native.Factor("L", m.a, m.ia, m.ja, precision="fp32_refined")
+    st = f.stats()
+    assert st["method"] == 2 and st["refine_steps"] >= 1
+    X = f.solve(B, nrhs=4)
+    for j in range(4):
+        assert relerr(X[j], orc.ldsol(m.a, m.ia, m.ja, B[j])) <= TOL
