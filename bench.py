@@ -9,7 +9,10 @@ entry in 2% of the rows, FP64 CSR with int32 indices, one right-hand side,
 right-hand side; b and x are resident in HBM when the timed region starts.
 
 Prints one JSON line.  `value` = rows*RHS solved per second over the timed
-steps.  `roofline` is for the dominant kernel (the block solve, two launches
+steps.  Other workloads (--workload): lap2d_256 (P=8), lap3d_128 (32 RHS,
+FP64 and FP32 + FP64 refinement), batched (4096 uneven banded matrices with
+1-8 RHS each, one graph launch per step), serving (64 resident matrices,
+Poisson arrivals of 1-4 RHS requests, 200 us batching windows).  `roofline` is for the dominant kernel (the block solve, two launches
 per step): algorithmic bytes per launch = 12 B per in-block nonzero (FP64
 value + int32 column) + 4 B per row pointer + 8 B read of b + 8 B write per
 row, divided by the kernel's average device time measured with HIP events on
@@ -89,12 +92,200 @@ def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
     return cpu, rel
 
 
+def reldiff(x, ref):
+    import numpy as np
+
+    return float(np.max(np.abs(x - ref)) / max(float(np.max(np.abs(ref))), 1e-300))
+
+
+def timed_solves(f, B, X, nr, stream, reps):
+    """Average device milliseconds of one solve (b, x resident; graph replays after the first two)."""
+    import torch
+
+    for _ in range(3):
+        f.solve(B.data_ptr(), X.data_ptr(), nrhs=nr, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        f.solve(B.data_ptr(), X.data_ptr(), nrhs=nr, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def emit(out):
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def base_line(args, value, ms_step, dtype, config):
+    return {"metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic (deterministic SplitMix64 generator)", "config": config}
+
+
+def bench_lap3d(args):
+    """3D 7-point Laplacian lower triangle on 128^3, 32 RHS: FP64 and FP32 + FP64 refinement."""
+    import numpy as np
+    import torch
+
+    from spike_pstrsv_amd import gen, native
+
+    m = gen.lap3d(128)
+    n, nr = m.n, 32
+    stream = torch.cuda.current_stream()
+    B = torch.from_numpy(gen.rhs(n, nr, seed=gen.SEED_BASE + 3).reshape(nr, n)).cuda()
+    res = {}
+    x64 = None
+    for prec in ("fp64", "fp32_refined"):
+        f = native.Factor("L", m.a, m.ia, m.ja, precision=prec)
+        st = f.stats()
+        X = torch.zeros_like(B)
+        ms = timed_solves(f, B, X, nr, stream, max(3, args.steps // 20))
+        Xh = X.cpu().numpy()
+        if x64 is None:
+            x64 = Xh
+        res[prec] = {"ms_per_solve": ms, "rows_rhs_per_s": n * nr / (ms * 1e-3),
+                     "rel_diff_vs_fp64_gpu": reldiff(Xh, x64),
+                     "method": "levels" if st["method"] == 2 else "spike", "levels": st["levels"],
+                     "refine_steps": st["refine_steps"], "preproc_s": st["preproc_seconds"]}
+        f.close()
+    best = max(res.values(), key=lambda r: r["rows_rhs_per_s"])
+    b_alg = 12 * m.nnz + 4 * (n + 1) + 16 * n * nr
+    out = base_line(args, best["rows_rhs_per_s"], best["ms_per_solve"], "f64",
+                    {"workload": "lap3d_128", "n": n, "nnz": m.nnz, "nrhs": nr, "uplo": "L"})
+    out["precisions"] = res
+    out["solve_roofline_frac"] = b_alg / (best["ms_per_solve"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    return emit(out)
+
+
+def bench_batched(args):
+    """4096 banded matrices, n log-uniform in [1000, 65536], 1-8 RHS each, one graph launch per solve."""
+    import numpy as np
+    import torch
+
+    from spike_pstrsv_amd import gen, native
+
+    specs, mats = gen.batched(4096)
+    ms_in = [(mm.a, mm.ia, mm.ja, s["nrhs"]) for s, mm in zip(specs, mats)]
+    stream = torch.cuda.current_stream()
+    res = {}
+    x64 = None
+    for prec in ("fp64", "fp32_refined"):
+        t0 = time.perf_counter()
+        bt = native.Batch("L", ms_in, precision=prec)
+        create_s = time.perf_counter() - t0
+        L = bt.length
+        b = np.empty(L)
+        for i, s in enumerate(specs):
+            b[bt.offsets[i]:bt.offsets[i + 1]] = gen.rhs(s["n"], s["nrhs"], seed=gen.SEED_BASE + 300 + i).ravel()
+        db = torch.from_numpy(b).cuda()
+        dx = torch.zeros_like(db)
+        for _ in range(2):
+            bt.solve(db.data_ptr(), dx.data_ptr(), stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        reps = max(3, args.steps // 20)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            bt.solve(db.data_ptr(), dx.data_ptr(), stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        x = dx.cpu().numpy()
+        if x64 is None:
+            x64 = x
+        worst = 0.0
+        for i in range(len(specs)):
+            seg = slice(bt.offsets[i], bt.offsets[i + 1])
+            worst = max(worst, reldiff(x[seg], x64[seg]))
+        st = bt.stats()
+        res[prec] = {"ms_per_solve": ms, "rows_rhs_per_s": L / (ms * 1e-3), "rel_diff_vs_fp64_gpu": worst,
+                     "create_s": create_s, "rows_per_partition": st["rows_per_partition"],
+                     "refine_steps": st["refine_steps"], "groups": st["groups"], "graph_nodes": st["graph_nodes"]}
+        bt.close()
+        del db, dx
+        torch.cuda.empty_cache()
+    best = max(res.values(), key=lambda r: r["rows_rhs_per_s"])
+    nnz = sum(mm.nnz for mm in mats)
+    rows = sum(s["n"] for s in specs)
+    rows_rhs = sum(s["n"] * s["nrhs"] for s in specs)
+    b_alg = 12 * nnz + 4 * (rows + len(specs)) + 16 * rows_rhs
+    out = base_line(args, best["rows_rhs_per_s"], best["ms_per_solve"], "f64",
+                    {"workload": "batched_4096", "matrices": len(specs), "rows": rows, "nnz": nnz,
+                     "rows_rhs": rows_rhs})
+    out["precisions"] = res
+    out["solve_roofline_frac"] = b_alg / (best["ms_per_solve"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    return emit(out)
+
+
+def bench_serving(args):
+    """64 resident matrices; Poisson arrivals at --rate req/s for --duration s; 1-4 RHS per request."""
+    import numpy as np
+    import torch
+
+    from spike_pstrsv_amd import gen, native
+
+    specs = gen.serving_specs()
+    srv = native.Server(window_us=200.0)
+    ids, ns = [], []
+    t0 = time.perf_counter()
+    for s in specs:
+        mm = gen.serving_matrix(s)
+        ids.append(srv.add("L", mm.a, mm.ia, mm.ja))
+        ns.append(mm.n)
+    setup_s = time.perf_counter() - t0
+    B = [torch.from_numpy(gen.rhs(nn, 4, seed=gen.SEED_BASE + 600 + i).reshape(4, nn)).cuda()
+         for i, nn in enumerate(ns)]
+    R = 8
+    X = [[torch.zeros(4, nn, dtype=torch.float64, device="cuda") for _ in range(R)] for nn in ns]
+    rng = np.random.default_rng(11)
+    srv.reset_stats()
+    for _ in range(200):                                   # requests served alone (closed loop)
+        i = int(rng.integers(len(ns)))
+        srv.wait(srv.submit(ids[i], int(rng.integers(1, 5)), B[i], X[i][0]))
+    alone = srv.stats()
+    srv.reset_stats()
+    nreq = int(args.rate * args.duration)
+    arr = np.cumsum(rng.exponential(1.0 / args.rate, nreq))
+    mids = rng.integers(len(ns), size=nreq)
+    nrs = rng.integers(1, 5, size=nreq)
+    t0 = time.perf_counter()
+    for k in range(nreq):
+        while time.perf_counter() - t0 < arr[k]:
+            pass
+        i = int(mids[k])
+        srv.submit(ids[i], int(nrs[k]), B[i], X[i][k % R])
+    submit_s = time.perf_counter() - t0
+    srv.drain()
+    st = srv.stats()
+    srv.close()
+    out = base_line(args, st["rows_rhs_per_s"], 1e3 * st["seconds"] / max(st["solves"], 1), "f64",
+                    {"workload": "serving_64", "matrices": len(ns), "rows": int(sum(ns)),
+                     "offered_req_per_s": args.rate, "duration_s": args.duration, "window_us": 200.0})
+    out["scaling"] = "weak"
+    out["latency_us"] = {"p50": st["p50_us"], "p99": st["p99_us"], "max": st["max_us"]}
+    out["alone_latency_us"] = {"p50": alone["p50_us"], "p99": alone["p99_us"]}
+    out["served_req_per_s"] = st["completed"] / st["seconds"] if st["seconds"] > 0 else None
+    out["serving"] = {k: st[k] for k in ("completed", "solves", "windows", "merged_rhs", "max_inflight",
+                                         "resident_bytes", "seconds")}
+    out["submit_seconds"] = submit_s
+    out["setup_seconds"] = setup_s
+    return emit(out)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="banded_1m")
+    ap.add_argument("--workload", default="banded_1m",
+                    choices=["banded_1m", "lap2d_256", "lap3d_128", "batched", "serving"])
+    ap.add_argument("--rate", type=float, default=20000.0, help="serving: offered requests per second")
+    ap.add_argument("--duration", type=float, default=0.5, help="serving: seconds of arrivals")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=None)
@@ -125,6 +316,12 @@ def main() -> int:
         return 1
     from spike_pstrsv_amd import gen, native
 
+    if args.workload == "lap3d_128":
+        return bench_lap3d(args)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     m, P, desc = workload(args.workload)
     n = m.n
     b = gen.rhs(n, 1, seed=gen.SEED_BASE + 2)
@@ -226,6 +423,25 @@ def main() -> int:
                        "mall_flushed_p50": statistics.median(fl)},
         "wall_s_timed": wall,
     }
+    # variants of the same matrix (not the metric line): 4 RHS, FP32 + refinement
+    # (FP32 results are compared with the FP64 GPU result of the same RHS)
+    var = {}
+    Bv = torch.from_numpy(gen.rhs(n, 4, seed=gen.SEED_BASE + 7).reshape(4, n)).cuda()
+    x64 = None
+    for prec, nr in (("fp64", 4), ("fp32_refined", 1), ("fp32_refined", 4)):
+        fv = fac if prec == "fp64" else native.Factor("L", m.a, m.ia, m.ja, partitions=P, precision=prec)
+        B1 = Bv[:nr].contiguous()
+        Xv = torch.zeros_like(B1)
+        ms_v = timed_solves(fv, B1, Xv, nr, stream, max(5, args.steps // 10))
+        xh = Xv.cpu().numpy()
+        if x64 is None:
+            x64 = xh
+        var[f"{prec}_nrhs{nr}"] = {"rows_rhs_per_s": n * nr / (ms_v * 1e-3), "ms": ms_v,
+                                   "rel_diff_vs_fp64_gpu": reldiff(xh, x64[:nr]),
+                                   "refine_steps": fv.stats()["refine_steps"]}
+        if fv is not fac:
+            fv.close()
+    out["variants"] = var
     if not args.no_cpu_baseline:
         cpu, rel = cpu_baseline(m, P, b, x_gpu, args.cpu_seconds)
         out["cpu_baseline"] = cpu
