@@ -113,6 +113,9 @@ struct spike_factor_s {
     float *d_a32 = nullptr, *d_invd32 = nullptr;
     double *d_dlt = nullptr;
     int lev_G = 1;
+    // interleaved right-hand sides / solutions of multi-RHS level solves
+    double *d_bi = nullptr, *d_xi = nullptr;
+    size_t il_cap = 0;
     // row blocks of the residual kernel (the partitions, or 1024-row blocks)
     int nres = 0;
     DevPart *d_res = nullptr;
@@ -348,7 +351,7 @@ int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
     std::vector<int> len(n);
     for (int i = 0; i < n; i++) len[i] = pl.ia[i + 1] - pl.ia[i] - 1;
     f->lev_G = f->opts.lanes_per_row > 0 ? std::min(f->opts.lanes_per_row, 16) : spike::choose_lanes(len);
-    f->max_nr = 8;
+    f->max_nr = 32;
     {
         DevPart whole{};
         whole.s = 0;
@@ -357,7 +360,7 @@ int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
         whole.mflip = pl.flip ? 1 : 0;
         if ((rc = upload_row_blocks(f, std::vector<DevPart>{whole}))) return rc;
     }
-    if (f->fp32 && ((rc = f->dalloc(&f->d_r, (size_t)n * 8)) || (rc = f->dalloc(&f->d_dlt, (size_t)n * 8)))) return rc;
+    if (f->fp32 && ((rc = f->dalloc(&f->d_r, (size_t)n * 32)) || (rc = f->dalloc(&f->d_dlt, (size_t)n)))) return rc;
     pl.a.clear(); pl.a.shrink_to_fit();
     pl.ja.clear(); pl.ja.shrink_to_fit();
     pl.steps = spike::StepStream();
@@ -794,10 +797,60 @@ spike::BlockArgs make_args(spike_factor_s *f, const double *b, int ldb, double *
 // device: x = M^-1 b, or x += M^-1 b with accum (refinement update).  ev
 // (optional, 5 events) brackets the phases; their device times are added to
 // ph[0..3].
+// Interleaved workspaces for level solves of up to nrhs right-hand sides
+// (allocated before any launch, so never inside a graph capture).
+int ensure_level_ws(spike_factor_s *f, int nrhs)
+{
+    if (f->method != SPIKE_METHOD_LEVELS || nrhs < 2) return SPIKE_OK;
+    int nrp = 2;
+    while (nrp < std::min(nrhs, 32)) nrp <<= 1;
+    const size_t need = (size_t)f->plan.n * nrp;
+    if (f->il_cap >= need) return SPIKE_OK;
+    if (f->d_bi) dfree(f, f->d_bi, sizeof(double) * f->il_cap);
+    if (f->d_xi) dfree(f, f->d_xi, sizeof(double) * f->il_cap);
+    f->d_bi = f->d_xi = nullptr;
+    f->il_cap = 0;
+    f->drop_graphs();
+    int rc;
+    if ((rc = f->dalloc(&f->d_bi, need)) || (rc = f->dalloc(&f->d_xi, need))) return rc;
+    f->il_cap = need;
+    return SPIKE_OK;
+}
+
 int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
                  hipEvent_t *ev, double *ph)
 {
     const int n = f->plan.n;
+    if (nr >= 2) {
+        // one lane per row and right-hand side on interleaved vectors
+        int nrp = 2;
+        while (nrp < nr) nrp <<= 1;
+        if ((size_t)n * nrp > f->il_cap) return fail(SPIKE_ERR_STATE, "level workspaces not allocated");
+        spike::LevelArgs la{};
+        la.rows = f->d_lrows;
+        la.ia = f->d_ia;
+        la.ja = f->d_ja;
+        la.val = f->d_a;
+        la.invd = f->d_inv_diag;
+        la.val32 = f->d_a32;
+        la.invd32 = f->d_invd32;
+        la.fp32 = f->fp32 ? 1 : 0;
+        const int flip = f->plan.flip ? 1 : 0;
+        if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+        HIP_TRY(spike::launch_interleave(n, nr, nrp, flip, b, ldb, f->d_bi, s));
+        for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++)
+            HIP_TRY(spike::launch_level_il(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], nrp, la, f->d_bi,
+                                           f->d_xi, s));
+        HIP_TRY(spike::launch_deinterleave(n, nr, nrp, flip, f->d_xi, x, ldx, accum, s));
+        if (ev) {
+            HIP_TRY(hipEventRecord(ev[1], s));
+            HIP_TRY(hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            ph[0] += ms;
+        }
+        return SPIKE_OK;
+    }
     spike::LevelArgs la{};
     la.rows = f->d_lrows;
     la.ia = f->d_ia;
@@ -1061,6 +1114,10 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
     HIP_TRY(hipGetDevice(&dev));
     if (dev != f->device) HIP_TRY(hipSetDevice(f->device));
     const bool bdev = is_device_ptr(b), xdev = is_device_ptr(x);
+    {
+        int rc = ensure_level_ws(f, nrhs);
+        if (rc) return rc;
+    }
     if (bdev && xdev && !timed && f->opts.use_graph) {
         int rc = solve_graph(f, nrhs, b, ldb, x, ldx, s);
         if (rc != 1) return rc;

@@ -726,6 +726,51 @@ __global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int *__res
     }
 }
 
+// Multi-RHS level step on interleaved vectors (v[i*NRP + r]): NRP lanes per
+// row, one per right-hand side, each summing the row's entries in order
+// (the entry's value and column are the same for all lanes of the row; the
+// x loads of one entry are one contiguous NRP*8-byte segment).
+template <int NRP, class T>
+__global__ __launch_bounds__(256) void k_level_il(int r0, int cnt, const int *__restrict__ rows,
+                                                  const int *__restrict__ ia, const int *__restrict__ ja,
+                                                  const T *__restrict__ val, const T *__restrict__ invd,
+                                                  const double *__restrict__ bi, double *xi)
+{
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int q = gid / NRP, r = gid % NRP;
+    if (q >= cnt) return;
+    const int row = rows[r0 + q];
+    const int e1 = ia[row + 1] - 1;   // the diagonal is last
+    T acc = (T)0;
+    for (int e = ia[row]; e < e1; e++) acc = fma(val[e], (T)xi[(size_t)ja[e] * NRP + r], acc);
+    xi[(size_t)row * NRP + r] = (double)(((T)bi[(size_t)row * NRP + r] - acc) * invd[row]);
+}
+
+// Interleave / de-interleave between the caller's column-major layout
+// (caller index flip ? n-1-i : i, leading dimension ld, nr columns) and
+// v[i*NRP + r] over lower-form rows i.  Columns r >= nr are filled with 0.
+__global__ __launch_bounds__(256) void k_interleave(int n, int nr, int nrp, int flip, const double *__restrict__ src,
+                                                    int ld, double *__restrict__ dst)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)n * nrp) return;
+    const int i = (int)(t / nrp), r = (int)(t % nrp);
+    const int gm = flip ? n - 1 - i : i;
+    dst[t] = r < nr ? src[(size_t)r * ld + gm] : 0.0;
+}
+// x (caller layout) = v, or x += v with accum.
+__global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int nrp, int flip, const double *__restrict__ v,
+                                                      double *x, int ld, int accum)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (size_t)n * nr) return;
+    const int r = (int)(t / n), gm = (int)(t % n);
+    const int i = flip ? n - 1 - gm : gm;
+    const double w = v[(size_t)i * nrp + r];
+    double *dst = x + (size_t)r * ld + gm;
+    *dst = accum ? *dst + w : w;
+}
+
 // x[:, t] += d[:, t] for t < nr (refinement update of the level method).
 __global__ __launch_bounds__(256) void k_axpy(int n, int nr, const double *__restrict__ d, int ldd, double *x, int ldx)
 {
@@ -971,6 +1016,51 @@ hipError_t launch_level(int r0, int cnt, const LevelArgs &a, hipStream_t s)
     case 16: return launch_level_nr<16>(r0, cnt, a, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int NRP>
+hipError_t launch_level_il_n(int r0, int cnt, const LevelArgs &a, const double *bi, double *xi, hipStream_t s)
+{
+    const dim3 grid((unsigned)(((size_t)cnt * NRP + 255) / 256)), block(256);
+    if (a.fp32)
+        hipLaunchKernelGGL((k_level_il<NRP, float>), grid, block, 0, s, r0, cnt, a.rows, a.ia, a.ja, a.val32, a.invd32,
+                           bi, xi);
+    else
+        hipLaunchKernelGGL((k_level_il<NRP, double>), grid, block, 0, s, r0, cnt, a.rows, a.ia, a.ja, a.val, a.invd,
+                           bi, xi);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_il(int r0, int cnt, int nrp, const LevelArgs &a, const double *bi, double *xi, hipStream_t s)
+{
+    if (cnt <= 0) return hipSuccess;
+    switch (nrp) {
+    case 2: return launch_level_il_n<2>(r0, cnt, a, bi, xi, s);
+    case 4: return launch_level_il_n<4>(r0, cnt, a, bi, xi, s);
+    case 8: return launch_level_il_n<8>(r0, cnt, a, bi, xi, s);
+    case 16: return launch_level_il_n<16>(r0, cnt, a, bi, xi, s);
+    case 32: return launch_level_il_n<32>(r0, cnt, a, bi, xi, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_interleave(int n, int nr, int nrp, int flip, const double *src, int ld, double *dst, hipStream_t s)
+{
+    const size_t tot = (size_t)n * nrp;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_interleave, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, nrp, flip, src, ld,
+                       dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v, double *x, int ld, int accum,
+                               hipStream_t s)
+{
+    const size_t tot = (size_t)n * nr;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, nrp, flip, v, x,
+                       ld, accum);
+    return hipGetLastError();
 }
 
 hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int ldx, hipStream_t s)

@@ -136,6 +136,13 @@ struct LevelArgs {
 };
 // Rows rows[r0 .. r0+cnt) (one level): x = (b - strict-lower part . x) / diag.
 hipError_t launch_level(int r0, int cnt, const LevelArgs &a, hipStream_t s);
+// Level step of nrp interleaved right-hand sides (v[i*nrp + r], nrp in
+// {2, 4, 8, 16, 32}): one lane per row and right-hand side.
+hipError_t launch_level_il(int r0, int cnt, int nrp, const LevelArgs &a, const double *bi, double *xi, hipStream_t s);
+// Column-major caller layout <-> interleaved lower-form layout.
+hipError_t launch_interleave(int n, int nr, int nrp, int flip, const double *src, int ld, double *dst, hipStream_t s);
+hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v, double *x, int ld, int accum,
+                               hipStream_t s);
 // x[:, t] += d[:, t], t < nr.
 hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int ldx, hipStream_t s);
 
