@@ -108,6 +108,7 @@ typedef struct {
                               /* ldx, nrhs) repeats the previous call's is     */
                               /* captured into a hipGraph once and replayed    */
                               /* (up to 16 cached per factor)                  */
+    int autotune;             /* 1: time the RHS tile candidates at creation   */
 } spike_options_t;
 
 /* Solve methods. */
@@ -171,7 +172,7 @@ typedef struct {
     double preproc_seconds;   /* wall time of spike_factor_create              */
     int reduced_sweeps;       /* Jacobi sweeps per solve, 0 = exact solver     */
     double jacobi_bound;      /* bound on the reduced solve's relative error   */
-    int rhs_per_launch;       /* right-hand sides sharing one block solve      */
+    int rhs_per_launch;       /* right-hand sides per solve launch sequence    */
     int precision;            /* SPIKE_PREC_* of the factor                    */
     int refine_steps;         /* refinement steps per solve (FP32_REFINED)     */
     double refine_rate;       /* calibrated contraction per refinement step    */
@@ -179,6 +180,8 @@ typedef struct {
     int method;               /* SPIKE_METHOD_SPIKE or SPIKE_METHOD_LEVELS     */
     int levels;               /* dependency levels of the whole matrix (0 if   */
                               /* not computed)                                 */
+    int rhs_tile[4];          /* right-hand sides per block-solve wave at 1,   */
+                              /* 2, 4, 8 slots (Spike method)                  */
 } spike_stats_t;
 
 int spike_factor_stats(spike_factor_t f, spike_stats_t *st);

@@ -73,6 +73,8 @@ struct spike_factor_s {
     int ring = 8;               // block-solve step records in flight
     bool cpl_fused = false;     // pass 2 can compute its coupling update itself
     bool fuse_at[4] = {};       // ... and does so at 1, 2, 4, 8 RHS slots
+    int tile_at[4] = {1, 1, 1, 1};  // RHS per wave at 1, 2, 4, 8 slots
+    int ncu = 1, lds_cu = 65536;
     int max_nr = 1;             // right-hand sides per block-solve launch
     int cmax = 0;               // most coupling entries of one partition
     double *d_y = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_bc = nullptr;
@@ -237,15 +239,38 @@ int kmax_limit_for_device()
     return std::min(lim, 16383);
 }
 
-// Right-hand sides per block-solve launch.  Every partition is one wavefront
-// that keeps its x segment (kmax+2 doubles per RHS, plus cmax coupling
-// products per RHS when pass 2 fuses its coupling update) in LDS, so the slot
-// count trades work per step against resident partitions per CU.  Relative
-// step cost at 1/2/4/8 slots, measured on gfx950 with 1024-row partitions:
-// 1.0, 1.22, 2.1, 4.2.  A launch costs rounds x step cost, where rounds =
-// ceil(P / (CUs x partitions resident per CU)); the chosen count maximises
-// RHS per unit cost (ties go to fewer slots).  Pass 2 fuses the coupling
-// update only where that keeps its round count.
+// Launch rounds of a block solve with slots columns in tiles of t per wave,
+// each wave holding `bytes` of LDS.
+long tile_rounds(const spike_factor_s *f, int slots, int t, size_t bytes)
+{
+    long per_cu = std::max<long>(1, (long)((size_t)f->lds_cu / std::max<size_t>(bytes, 1)));
+    per_cu = std::min<long>(per_cu, 32);
+    const long waves = (long)f->plan.P * (slots / t);
+    return (waves + (long)f->ncu * per_cu - 1) / ((long)f->ncu * per_cu);
+}
+
+// Whether pass 2 fuses its coupling update at this slot count and tile: the
+// coupling products must fit the LDS next to the x segment without costing a
+// launch round.
+bool fuse_ok(const spike_factor_s *f, int slots, int t)
+{
+    const size_t lim = spike::block_solve_lds_limit();
+    const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
+    const size_t x_per = (size_t)(f->plan.kmax + 2) * vb, c_per = (size_t)f->cmax * vb;
+    if (!f->cpl_fused || (x_per + c_per) * t > lim) return false;
+    return tile_rounds(f, slots, t, (x_per + c_per) * t) == tile_rounds(f, slots, t, x_per * t);
+}
+
+// Right-hand sides per block-solve launch (chunk of up to 8 columns) and per
+// wave (the tile).  Every partition is solved by nr_slots / tile waves, each
+// keeping its x segment (kmax+2 values per RHS of its tile, plus cmax coupling
+// products per RHS when pass 2 fuses its coupling update) in LDS, so the tile
+// trades work per step against resident waves per CU.  Model: relative step
+// cost at tile 1/2/4/8 (measured on gfx950 with 1024-row partitions) 1.0,
+// 1.22, 2.1, 4.2; a launch costs rounds x step cost, rounds = ceil(waves /
+// (CUs x waves resident per CU)).  The model gives the default tile of each
+// slot count; the autotuner (tune_tiles) replaces it by measurement.  Pass 2
+// fuses the coupling update only where that keeps its round count.
 void choose_rhs_slots(spike_factor_s *f)
 {
     const Plan &pl = f->plan;
@@ -256,39 +281,98 @@ void choose_rhs_slots(spike_factor_s *f)
         lds_cu <= 0)
         lds_cu = (int)lim;
     (void)hipGetLastError();
+    f->ncu = ncu;
+    f->lds_cu = lds_cu;
     const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
     const size_t x_per = (size_t)(pl.kmax + 2) * vb;
     const size_t c_per = (size_t)f->cmax * vb;
     f->cpl_fused = x_per + c_per <= lim;
-    auto rounds = [&](size_t bytes) -> long {
-        long per_cu = std::max<long>(1, (long)((size_t)lds_cu / std::max<size_t>(bytes, 1)));
-        per_cu = std::min<long>(per_cu, 32);
-        return (pl.P + (long)ncu * per_cu - 1) / ((long)ncu * per_cu);
-    };
     static const double step_cost[4] = {1.0, 1.22, 2.1, 4.2};
-    double best = -1.0;
-    f->max_nr = 1;
     for (int li = 0; li < 4; li++) {
+        f->tile_at[li] = 0;
+        f->fuse_at[li] = false;
+    }
+    if (f->batch_slots > 0) {
+        // a batch group: one tile = its slot count
+        const int li = __builtin_ctz((unsigned)f->batch_slots);
+        if (x_per * f->batch_slots > lim) {
+            f->max_nr = 0;
+            return;
+        }
+        f->tile_at[li] = f->batch_slots;
+        f->fuse_at[li] = fuse_ok(f, f->batch_slots, f->batch_slots);
+        f->max_nr = f->batch_slots;
+        return;
+    }
+    f->max_nr = pl.steps.fwd ? 1 : 8;
+    for (int li = 0; li < 4 && (1 << li) <= f->max_nr; li++) {
         const int c = 1 << li;
-        if (c > 1 && pl.steps.fwd) break;
-        if (x_per * c > lim) break;
-        if (f->batch_slots > 0 && c != f->batch_slots) {
-            if (c > f->batch_slots) break;
-            continue;
+        double best = -1.0;
+        for (int ti = 0; ti <= li; ti++) {
+            const int t = 1 << ti;
+            if (x_per * t > lim) break;
+            const double thr = c / (step_cost[ti] * (double)tile_rounds(f, c, t, x_per * t));
+            if (thr > best * 1.02) {
+                best = thr;
+                f->tile_at[li] = t;
+            }
         }
-        const long r1 = rounds(x_per * c);
-        const bool fuse = f->cpl_fused && (x_per + c_per) * c <= lim && rounds((x_per + c_per) * c) == r1;
-        f->fuse_at[li] = fuse;
-        const double t = step_cost[li] * (double)(r1 + r1) + (fuse ? 0.0 : 0.05);
-        const double thr = c / t;
-        if (thr > best * 1.02) {
-            best = thr;
-            f->max_nr = c;
-        }
+        if (f->tile_at[li] == 0) f->tile_at[li] = 1;
+        f->fuse_at[li] = fuse_ok(f, c, f->tile_at[li]);
     }
 }
 
 int calibrate_refinement(spike_factor_s *f);
+int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
+               hipEvent_t *ev, double *ph);
+
+// Autotuner of the RHS tile: for 2, 4 and 8 slots, every tile that fits the
+// LDS is timed over two whole solves (after one warm-up) on zero right-hand
+// sides; the fastest replaces the model's choice (ties within 3% keep the
+// smaller tile).
+int tune_tiles(spike_factor_s *f)
+{
+    const int n = f->plan.n;
+    const size_t lim = spike::block_solve_lds_limit();
+    const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
+    const size_t x_per = (size_t)(f->plan.kmax + 2) * vb;
+    double *db = nullptr, *dx = nullptr;
+    int rc;
+    if ((rc = f->dalloc(&db, (size_t)n * 8)) || (rc = f->dalloc(&dx, (size_t)n * 8))) return rc;
+    HIP_TRY(hipMemset(db, 0, sizeof(double) * (size_t)n * 8));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int li = 1; li < 4 && (1 << li) <= f->max_nr && !rc; li++) {
+        const int c = 1 << li;
+        int best_t = f->tile_at[li];
+        float best_ms = 1e30f;
+        for (int ti = 0; ti <= li && !rc; ti++) {
+            const int t = 1 << ti;
+            if (x_per * t > lim) break;
+            f->tile_at[li] = t;
+            f->fuse_at[li] = fuse_ok(f, c, t);
+            if ((rc = solve_pass(f, db, n, dx, n, c, 0, nullptr, nullptr, nullptr))) break;
+            (void)hipEventRecord(e0, nullptr);
+            for (int r = 0; r < 2 && !rc; r++) rc = solve_pass(f, db, n, dx, n, c, 0, nullptr, nullptr, nullptr);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms * 0.97f) {
+                best_ms = ms;
+                best_t = t;
+            }
+        }
+        f->tile_at[li] = best_t;
+        f->fuse_at[li] = fuse_ok(f, c, best_t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(f, db, sizeof(double) * (size_t)n * 8);
+    dfree(f, dx, sizeof(double) * (size_t)n * 8);
+    return rc;
+}
 
 // Automatic partition count: partitions of about 16x the coupling bandwidth
 // (95th percentile of the distance from a row to its farthest column, so rare
@@ -732,6 +816,8 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.precision = f->opts.precision;
     st.method = SPIKE_METHOD_SPIKE;
     st.levels = f->nlev_full;
+    if (f->batch_slots == 0 && f->opts.autotune && !f->steps.fwd && (rc = tune_tiles(f))) return rc;
+    for (int li = 0; li < 4; li++) st.rhs_tile[li] = f->tile_at[li];
     if (f->fp32) {
         if ((rc = upload_row_blocks(f, hp))) return rc;
         if ((rc = calibrate_refinement(f))) return rc;
@@ -778,6 +864,7 @@ spike::BlockArgs make_args(spike_factor_s *f, const double *b, int ldb, double *
     ba.ring = f->ring;
     ba.inv_diag = f->d_inv_diag;
     ba.cpl_fused = f->fuse_at[__builtin_ctz((unsigned)slots)] ? 1 : 0;
+    ba.tile = std::max(1, f->tile_at[__builtin_ctz((unsigned)slots)]);
     ba.cmax = f->cmax;
     ba.cpl_ptr = f->d_cpl_ptr;
     ba.cpl_col = f->d_cpl_col;
@@ -1677,6 +1764,7 @@ void spike_options_default(spike_options_t *o)
     o->refine_tol = 0.0;
     o->method = SPIKE_METHOD_AUTO;
     o->use_graph = 1;
+    o->autotune = 1;
 }
 
 int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, const int *ia, const int *ja,

@@ -157,7 +157,8 @@ __device__ __forceinline__ RhsMap rhs_map(const BlockArgs &A, const DevPart &pt)
 // then sums its own products in entry order, so the result does not depend
 // on the lane count.  Right-hand sides r >= nr reuse column nr-1.
 template <int NR, class T>
-__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, T *prod, bool scaled)
+__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, T *prod, bool scaled,
+                                          int roff, int RS)
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k;
     const RhsMap m = rhs_map(A, pt);
@@ -165,7 +166,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
         const int g = s + i;
         const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
 #pragma unroll
-        for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(r, g) * sc;
+        for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, g) * sc;
     }
     if (pt.ccnt == 0) return;
     if (A.cpl_fused) {
@@ -174,7 +175,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
 #pragma unroll 4
         for (int e = e0 + lane; e < e1; e += 64) {
             const T w = cv[e];
-            const double *xc = A.xr + (size_t)A.cpl_col[e] * NR;
+            const double *xc = A.xr + (size_t)A.cpl_col[e] * RS + roff;
 #pragma unroll
             for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * (T)xc[r];
         }
@@ -189,7 +190,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
 #pragma unroll
                 for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
 #pragma unroll
-            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = ((T)m.bv(r, g) - acc[r]) * sc;
+            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = ((T)m.bv(roff + r, g) - acc[r]) * sc;
         }
     } else {
         __syncthreads();
@@ -197,13 +198,17 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
             const int g = A.cpl_rows[pt.cbeg + c];
             const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
 #pragma unroll
-            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = (T)A.bc[(size_t)(pt.cbeg + c) * NR + r] * sc;
+            for (int r = 0; r < NR; r++)
+                xs[(g - s) * NR + r] = (T)A.bc[(size_t)(pt.cbeg + c) * RS + roff + r] * sc;
         }
     }
 }
 
 // Block solve with NR right-hand sides sharing each step record; x segments
-// interleaved in LDS as xs[i*NR + r] (i < k+2, slot k holds 0).
+// interleaved in LDS as xs[i*NR + r] (i < k+2, slot k holds 0).  A launch of
+// nr_slots right-hand sides runs nr_slots / NR waves per partition
+// (blockIdx.y picks columns roff .. roff+NR-1); the reduced vectors hold
+// nr_slots interleaved values per index.
 template <int G, int M, int D, int PASS, int NR, class T>
 __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 {
@@ -220,15 +225,16 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     if (PASS == 1 && pt.hi1 < 0) return;
     if (PASS == 2 && pt.f >= k) return;
     const RhsMap m = rhs_map(A, pt);
+    const int roff = blockIdx.y * NR, RS = A.nr_slots;
 
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
         for (int i = lane; i < k; i += 64) {
 #pragma unroll
-            for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(r, s + i);
+            for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, s + i);
         }
     } else {
-        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false);
+        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false, roff, RS);
     }
     if (lane == 0)
 #pragma unroll
@@ -326,7 +332,7 @@ epilogue:
         for (int q = lane; q < pt.dcnt; q += 64) {
             const int i = A.dep_rows[pt.dbeg + q] - s;
 #pragma unroll
-            for (int r = 0; r < NR; r++) A.xred[(size_t)(pt.dbeg + q) * NR + r] = (double)xs[i * NR + r];
+            for (int r = 0; r < NR; r++) A.xred[(size_t)(pt.dbeg + q) * RS + roff + r] = (double)xs[i * NR + r];
         }
     // pass 2 writes the solution; without coupling, pass 1's result is final
     if (PASS == 2 || pt.f >= k)
@@ -334,7 +340,7 @@ epilogue:
             const int gm = m.at(s + i);
 #pragma unroll
             for (int r = 0; r < NR; r++)
-                if (r < m.nr) put_x(m.x, (size_t)r * m.ldx + gm, (double)xs[i * NR + r], A.accum);
+                if (roff + r < m.nr) put_x(m.x, (size_t)(roff + r) * m.ldx + gm, (double)xs[i * NR + r], A.accum);
         }
 }
 
@@ -367,7 +373,7 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
             xs[r] = m.bv(0, g) * invd[g];
         }
     } else {
-        pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + 2, true);
+        pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + 2, true, 0, 1);
     }
     if (lane == 0) {
         xs[k] = 0.0;
@@ -458,7 +464,7 @@ hipError_t launch_gdt(int P, const BlockArgs &a, hipStream_t s)
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(P), dim3(64), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3(P, a.nr_slots / NR), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
@@ -471,7 +477,8 @@ hipError_t launch_gd(int P, const BlockArgs &a, hipStream_t s)
 template <int G, int PASS>
 hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
 {
-    switch (a.nr_slots) {
+    if (a.tile < 1 || a.tile > a.nr_slots || a.nr_slots % a.tile) return hipErrorInvalidValue;
+    switch (a.tile) {
     case 1: return launch_gd<G, 8, PASS, 1>(P, a, s);
     case 2: return launch_gd<G, 8, PASS, 2>(P, a, s);
     case 4: return launch_gd<G, 8, PASS, 4>(P, a, s);

@@ -70,6 +70,9 @@ struct BlockArgs {
     const float *cpl_val32;
     // 1: the solve's result is added to x (refinement update x += d)
     int accum;
+    // right-hand sides per block-solve wave (divides nr_slots): a launch runs
+    // nr_slots / tile waves per partition
+    int tile;
 };
 
 // Pass 1 (y = D^-1 b, written at the dependents; also the solution of

@@ -40,7 +40,7 @@ class Options(C.Structure):
     _fields_ = [("partitions", C.c_int), ("drop_tol", C.c_double), ("precision", C.c_int),
                 ("lanes_per_row", C.c_int), ("verbose", C.c_int), ("reduced_solver", C.c_int),
                 ("refine_steps", C.c_int), ("refine_tol", C.c_double), ("method", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("autotune", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -51,10 +51,12 @@ class Stats(C.Structure):
                 ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double),
                 ("rhs_per_launch", C.c_int), ("precision", C.c_int), ("refine_steps", C.c_int),
                 ("refine_rate", C.c_double), ("refine_est_err", C.c_double), ("method", C.c_int),
-                ("levels", C.c_int)]
+                ("levels", C.c_int), ("rhs_tile", C.c_int * 4)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["rhs_tile"] = list(self.rhs_tile)
+        return d
 
 
 class BatchMatrix(C.Structure):
