@@ -141,6 +141,16 @@ void spike_options_default(spike_options_t *opts);
 int spike_factor_create(spike_factor_t *f, char uplo, int n, const double *a,
                         const int *ia, const int *ja, const spike_options_t *opts);
 
+/* Same as spike_factor_create with the caller's partition boundaries:
+ * nparts partitions of rows [starts[t], starts[t+1]) in the caller's row
+ * order (0 = starts[0] < ... < starts[nparts] = n), each no longer than the
+ * device's LDS limit (16383 rows in FP64).  Replaces the reference's
+ * IS_MATRIX_PARTITIONED path (src/matrix_partitioning.c:23, :34-35), where
+ * partition_starts/partition_ends come from the caller (e.g. a reordering);
+ * the factor always uses the Spike method. */
+int spike_factor_create_partitioned(spike_factor_t *f, char uplo, int n, const double *a, const int *ia,
+                                    const int *ja, int nparts, const int *starts, const spike_options_t *opts);
+
 /* Solves A X = B for nrhs right-hand sides (column-major, leading dimensions
  * ldb/ldx >= n).  b and x may be host or device pointers.  stream is a
  * hipStream_t (NULL = default stream); for device pointers the call is

@@ -2,6 +2,7 @@
 // (reference src/spike_pstrsv.h:7-38) over a device-resident factor object,
 // plus the extension API declared in include/spike_pstrsv.h.
 #include "spike_pstrsv.h"
+#include "spike_pstrsv_globals.h"
 
 #include <hip/hip_runtime_api.h>
 
@@ -20,6 +21,32 @@
 
 using spike::DevPart;
 using spike::Plan;
+
+// Globals read by the reference's timing harness (include/spike_pstrsv_globals.h).
+extern "C" {
+char PSTRSV_VERBOSE = 0;
+char FORCE_LOAD_BALANCE = 0;
+char IS_MATRIX_PARTITIONED = 0;
+double memoryAlloc_timer_pstrsv3 = 0;
+double preprocessD_timer_pstrsv3 = 0;
+double preprocessR_timer_pstrsv3 = 0;
+double setDependents_timer_pstrsv3 = 0;
+double preprocessS_timer_pstrsv3 = 0;
+double preprocessMappings_timer_pstrsv3 = 0;
+double produceCompressedR_timer_pstrsv3 = 0;
+double produceS_timer_pstrsv3 = 0;
+double produceReducedSystem_timer_pstrsv3 = 0;
+double init_vars_pstrsv3 = 0;
+double reduced_solver_timer_pstrsv3 = 0;
+double temp_timer[MAXTHREADCOUNT];
+double waitBefore_timer_pstrsv3[MAXTHREADCOUNT];
+double computeY_timer[MAXTHREADCOUNT];
+double solveIndependent_timer[MAXTHREADCOUNT];
+double subtract_Rx_timer[MAXTHREADCOUNT];
+double waitAfter_timer_pstrsv3[MAXTHREADCOUNT];
+double solveD2_timer[MAXTHREADCOUNT];
+double solveD_timer[MAXTHREADCOUNT];
+}
 
 namespace {
 
@@ -109,6 +136,7 @@ struct spike_factor_s {
     // host delimits the launches), the lower-form CSR and 1/diag on the
     // device, a correction workspace for refinement
     int method = SPIKE_METHOD_SPIKE;
+    bool caller_parts = false;  // partitions given by the caller: Spike only
     int nlev_full = 0;
     std::vector<int> lev_ptr;
     int *d_lrows = nullptr;
@@ -512,7 +540,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         const double t_lev = nl * 3.0 + (double)pl.nnz * 12.0 / 4e6;
         const double t_spk = 2.0 * pl.steps.max_steps * 0.19 + (nd > 0 ? 7.0 * (2.5 + nd * 25.0 * 12.0 / 4e6) : 0.0);
         int method = f->opts.method;
-        if (!f->mats.empty()) method = SPIKE_METHOD_SPIKE;
+        if (!f->mats.empty() || f->caller_parts) method = SPIKE_METHOD_SPIKE;
         if (method == SPIKE_METHOD_AUTO)
             method = (nd > n / 8.0 || t_lev < t_spk) ? SPIKE_METHOD_LEVELS : SPIKE_METHOD_SPIKE;
         f->method = method;
@@ -1281,6 +1309,30 @@ int legacy_partitions(int n, int nthreads)
     return std::min(P, std::max(n, 1));
 }
 
+// Copies the library's phase timers into the harness globals.
+void sync_harness_timers()
+{
+    preprocessD_timer_pstrsv3 = g_timers[1];
+    produceS_timer_pstrsv3 = g_timers[2];
+    produceReducedSystem_timer_pstrsv3 = g_timers[3];
+    reduced_solver_timer_pstrsv3 = g_timers[6];
+    computeY_timer[0] = g_timers[5];
+    subtract_Rx_timer[0] = g_timers[7];
+    solveD2_timer[0] = g_timers[8];
+}
+
+void reset_harness_timers()
+{
+    for (double &v : g_timers) v = 0.0;
+    memoryAlloc_timer_pstrsv3 = preprocessD_timer_pstrsv3 = preprocessR_timer_pstrsv3 = 0.0;
+    setDependents_timer_pstrsv3 = preprocessS_timer_pstrsv3 = preprocessMappings_timer_pstrsv3 = 0.0;
+    produceCompressedR_timer_pstrsv3 = produceS_timer_pstrsv3 = produceReducedSystem_timer_pstrsv3 = 0.0;
+    init_vars_pstrsv3 = reduced_solver_timer_pstrsv3 = 0.0;
+    for (int i = 0; i < MAXTHREADCOUNT; i++)
+        temp_timer[i] = waitBefore_timer_pstrsv3[i] = computeY_timer[i] = solveIndependent_timer[i] =
+            subtract_Rx_timer[i] = waitAfter_timer_pstrsv3[i] = solveD2_timer[i] = solveD_timer[i] = 0.0;
+}
+
 void legacy_preproc(double *a, int *ia, int *ja, int n, int nthreads, char uplo, const char *who)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1298,6 +1350,7 @@ void legacy_preproc(double *a, int *ia, int *ja, int n, int nthreads, char uplo,
         return;
     }
     g_cur = f;
+    if (PSTRSV_VERBOSE) spike_factor_log(f);
 }
 
 void legacy_solve(char uplo, int m, double *b, double *x, const char *who, bool timed)
@@ -1795,6 +1848,45 @@ int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, 
     return SPIKE_OK;
 }
 
+int spike_factor_create_partitioned(spike_factor_t *out, char uplo, int n, const double *a, const int *ia,
+                                    const int *ja, int nparts, const int *starts, const spike_options_t *opts)
+{
+    if (!out) return fail(SPIKE_ERR_ARG, "factor pointer is NULL");
+    *out = nullptr;
+    if (uplo == 'l') uplo = 'L';
+    if (uplo == 'u') uplo = 'U';
+    if (uplo != 'L' && uplo != 'U') return fail(SPIKE_ERR_ARG, "uplo must be L or U");
+    if (n < 1) return fail(SPIKE_ERR_ARG, "n must be >= 1");
+    if (!a || !ia || !ja || !starts) return fail(SPIKE_ERR_ARG, "a, ia, ja and starts must be non-NULL");
+    if (nparts < 1 || nparts > n) return fail(SPIKE_ERR_ARG, "nparts must be in [1, n]");
+    if (starts[0] != 0 || starts[nparts] != n) return fail(SPIKE_ERR_ARG, "starts must begin at 0 and end at n");
+    for (int t = 0; t < nparts; t++)
+        if (starts[t + 1] <= starts[t]) return fail(SPIKE_ERR_ARG, "starts must be strictly increasing");
+    spike_options_t o;
+    spike_options_default(&o);
+    if (opts) o = *opts;
+    if (o.precision != SPIKE_PREC_FP64 && o.precision != SPIKE_PREC_FP32_REFINED)
+        return fail(SPIKE_ERR_ARG, "unknown precision mode");
+    // boundaries of the lower form: 'U' rows are reversed, so partition t of
+    // the caller is partition nparts-1-t there
+    std::vector<int> sl(nparts + 1);
+    for (int t = 0; t <= nparts; t++) sl[t] = uplo == 'U' ? n - starts[nparts - t] : starts[t];
+    spike_factor_s *f = new spike_factor_s();
+    f->opts = o;
+    f->caller_parts = true;
+    int rc = factor_build(f, uplo, n, a, ia, ja, &sl);
+    if (rc != SPIKE_OK) {
+        std::string keep = t_err;
+        f->release();
+        delete f;
+        t_err = keep;
+        return rc;
+    }
+    *out = f;
+    t_err.clear();
+    return SPIKE_OK;
+}
+
 int spike_factor_solve(spike_factor_t f, int nrhs, const double *b, int ldb, double *x, int ldx, void *stream)
 {
     if (!f) return fail(SPIKE_ERR_STATE, "factor is NULL");
@@ -1907,8 +1999,9 @@ void spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthrea
 void _spike_pstrsv_preproc(double *a, MKL_INT *ia, MKL_INT *ja, int n, int nthreads, const char uplo)
 {
     // timers restart per preprocess, like initTimers_PSTRSV3 (ops.c:108-120)
-    for (double &v : g_timers) v = 0.0;
+    reset_harness_timers();
     legacy_preproc(a, ia, ja, n, nthreads, uplo, "_spike_pstrsv_preproc");
+    sync_harness_timers();
 }
 
 void spike_pstrsv(const char uplo, int m, double *a, MKL_INT *ia, MKL_INT *ja, double *b, double *x, int nthreads)
@@ -1927,6 +2020,7 @@ void _spike_pstrsv_solve(const char uplo, int m, double *b, double *x, int nthre
 {
     (void)nthreads;
     legacy_solve(uplo, m, b, x, "_spike_pstrsv_solve", true);
+    sync_harness_timers();
 }
 
 // ---------------------------------------------------------------- batch API

@@ -972,11 +972,14 @@ hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int
 {
     if (P <= 0 || nr_max <= 0) return hipSuccess;
     const dim3 grid(P), block(256);
-    for (int c0 = 0; c0 < nr_max; c0 += 4) {
-        if (nr_max - c0 == 1)
+    for (int c0 = 0; c0 < nr_max; c0 += 8) {
+        const int left = nr_max - c0;
+        if (left == 1)
             hipLaunchKernelGGL(k_residual<1>, grid, block, 0, s, parts, ia, ja, a_, a, r, ldr, c0);
-        else
+        else if (left <= 4)
             hipLaunchKernelGGL(k_residual<4>, grid, block, 0, s, parts, ia, ja, a_, a, r, ldr, c0);
+        else
+            hipLaunchKernelGGL(k_residual<8>, grid, block, 0, s, parts, ia, ja, a_, a, r, ldr, c0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

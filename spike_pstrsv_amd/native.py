@@ -24,7 +24,8 @@ ERRORS = {1: "SPIKE_ERR_ARG", 2: "SPIKE_ERR_MATRIX", 3: "SPIKE_ERR_DEVICE", 4: "
 # Every symbol declared in include/spike_pstrsv.h.
 EXPORTED = [
     "spike_pstrsv_preproc", "_spike_pstrsv_preproc", "spike_pstrsv", "spike_pstrsv_solve", "_spike_pstrsv_solve",
-    "spike_options_default", "spike_factor_create", "spike_factor_solve", "spike_factor_solve_profiled",
+    "spike_options_default", "spike_factor_create", "spike_factor_create_partitioned", "spike_factor_solve",
+    "spike_factor_solve_profiled",
     "spike_factor_destroy",
     "spike_factor_stats", "spike_factor_log", "spike_analyze", "spike_pstrsv_set_partitions",
     "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
@@ -120,6 +121,8 @@ lib.spike_options_default.argtypes = [C.POINTER(Options)]
 lib.spike_options_default.restype = None
 lib.spike_factor_create.argtypes = [C.POINTER(_vp), _c, _i, _vp, _vp, _vp, C.POINTER(Options)]
 lib.spike_factor_create.restype = _i
+lib.spike_factor_create_partitioned.argtypes = [C.POINTER(_vp), _c, _i, _vp, _vp, _vp, _i, _vp, C.POINTER(Options)]
+lib.spike_factor_create_partitioned.restype = _i
 lib.spike_factor_solve.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
 lib.spike_factor_solve.restype = _i
 lib.spike_factor_solve_profiled.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp, C.POINTER(_d)]
@@ -231,14 +234,20 @@ class Factor:
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
                  lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto",
                  precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0,
-                 method: str = "auto"):
+                 method: str = "auto", starts=None):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()
         o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver, precision, refine_steps,
                     refine_tol, method)
-        _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
-                                       ja.ctypes.data, C.byref(o)))
+        if starts is None:
+            _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
+                                           ja.ctypes.data, C.byref(o)))
+        else:
+            st = np.ascontiguousarray(starts, dtype=np.int32)
+            _check(lib.spike_factor_create_partitioned(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data,
+                                                       ia.ctypes.data, ja.ctypes.data, len(st) - 1, st.ctypes.data,
+                                                       C.byref(o)))
         self.handle = h.value
 
     def solve(self, b, x=None, nrhs: int = 1, ldb: int | None = None, ldx: int | None = None, stream: int = 0):

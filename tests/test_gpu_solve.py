@@ -174,3 +174,51 @@ def test_deterministic(native):
     x1 = f.solve(b)
     x2 = f.solve(b)
     assert np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_caller_partitions(native, uplo):
+    """Caller-provided partition boundaries (uneven, in the caller's row order)."""
+    m = gen.banded(1 << 15, seed=gen.SEED_BASE + 80)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    rng = np.random.default_rng(3)
+    cuts = np.sort(rng.choice(np.arange(1, m.n), size=23, replace=False))
+    starts = np.concatenate([[0], cuts, [m.n]]).astype(np.int32)
+    assert np.max(np.diff(starts)) <= 16000
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 81)
+    f = native.Factor(uplo, m.a, m.ia, m.ja, starts=starts)
+    assert f.stats()["partitions"] == 24 and f.stats()["method"] == 1
+    assert relerr(f.solve(b), orc.trsv(uplo, m.a, m.ia, m.ja, b)) <= TOL
+    # the formula's boundaries reproduce the default factor bit for bit
+    P = 32
+    k = m.n // P
+    fs = native.Factor(uplo, m.a, m.ia, m.ja, starts=[k * t for t in range(P)] + [m.n])
+    fd = native.Factor(uplo, m.a, m.ia, m.ja, partitions=P)
+    if uplo == "L":
+        assert np.array_equal(fs.solve(b), fd.solve(b))
+    else:
+        assert relerr(fs.solve(b), fd.solve(b)) <= TOL
+
+
+def test_acceptance_program(tmp_path):
+    """The C acceptance program (reference call sequence, harness globals)."""
+    import subprocess
+    from pathlib import Path
+
+    import torch
+
+    assert torch.cuda.is_available()
+    from spike_pstrsv_amd import native
+
+    root = Path(__file__).resolve().parent.parent
+    exe = tmp_path / "example_usage"
+    libdir = str(native.LIB_PATH.parent)
+    r = subprocess.run(["gcc", "-O2", "-std=c99", "-I", str(root / "include"),
+                        str(root / "tests" / "c" / "example_usage.c"), "-L", libdir, "-lspike_pstrsv",
+                        f"-Wl,-rpath,{libdir}", "-lm", "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(run.stdout[-3000:])
+    assert run.returncode == 0 and "PASSED" in run.stdout, run.stdout + run.stderr
+    assert "PSTRSV" in run.stdout or "partition" in run.stdout.lower()

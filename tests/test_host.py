@@ -133,3 +133,41 @@ def test_banded_1m_expected_size():
     m = gen.banded(1 << 20)
     # E[nnz] = 17,846,628 by the generator's definition; realised within 0.1%
     assert abs(m.nnz - 17_846_628) / 17_846_628 < 1e-3
+
+
+def globals_declared():
+    text = (ROOT / "include" / "spike_pstrsv_globals.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"extern\s+(char|double)\s+(\w+)(\[MAXTHREADCOUNT\])?\s*;", text)
+
+
+def test_harness_globals_exported():
+    """Every global of include/spike_pstrsv_globals.h is a data symbol of the
+    library (the names the reference's harness reads)."""
+    from spike_pstrsv_amd import native
+
+    lib = ctypes.CDLL(str(native.LIB_PATH))
+    decl = globals_declared()
+    assert len(decl) >= 21
+    for ctype, name, arr in decl:
+        t = ctypes.c_char if ctype == "char" else ctypes.c_double
+        if arr:
+            t = t * 128
+        v = t.in_dll(lib, name)
+        assert v is not None
+
+
+def test_acceptance_program_compiles_and_links(tmp_path):
+    """The C acceptance program (reference call sequence + harness globals)
+    builds against include/ and links against libspike_pstrsv.so."""
+    import subprocess
+
+    from spike_pstrsv_amd import native
+
+    exe = tmp_path / "example_usage"
+    libdir = str(native.LIB_PATH.parent)
+    r = subprocess.run(["gcc", "-O2", "-Wall", "-Werror", "-std=c99", "-I", str(ROOT / "include"),
+                        str(ROOT / "tests" / "c" / "example_usage.c"), "-L", libdir, "-lspike_pstrsv",
+                        f"-Wl,-rpath,{libdir}", "-lm", "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert exe.exists()
