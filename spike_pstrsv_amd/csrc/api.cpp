@@ -261,9 +261,9 @@ void dfree(spike_factor_s *f, void *p, size_t bytes)
 
 int kmax_limit_for_device()
 {
-    // x segment of kmax+2 doubles (zero slot and scratch slot after the rows)
+    // x segment of kmax + XS_PAD doubles (spare slots after the rows)
     size_t lds = spike::block_solve_lds_limit();
-    int lim = (int)(lds / sizeof(double)) - 2;
+    int lim = (int)(lds / sizeof(double)) - spike::XS_PAD;
     return std::min(lim, 16383);
 }
 
@@ -284,14 +284,14 @@ bool fuse_ok(const spike_factor_s *f, int slots, int t)
 {
     const size_t lim = spike::block_solve_lds_limit();
     const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
-    const size_t x_per = (size_t)(f->plan.kmax + 2) * vb, c_per = (size_t)f->cmax * vb;
+    const size_t x_per = (size_t)(f->plan.kmax + spike::XS_PAD) * vb, c_per = (size_t)f->cmax * vb;
     if (!f->cpl_fused || (x_per + c_per) * t > lim) return false;
     return tile_rounds(f, slots, t, (x_per + c_per) * t) == tile_rounds(f, slots, t, x_per * t);
 }
 
 // Right-hand sides per block-solve launch (chunk of up to 8 columns) and per
 // wave (the tile).  Every partition is solved by nr_slots / tile waves, each
-// keeping its x segment (kmax+2 values per RHS of its tile, plus cmax coupling
+// keeping its x segment (kmax+XS_PAD values per RHS of its tile, plus cmax coupling
 // products per RHS when pass 2 fuses its coupling update) in LDS, so the tile
 // trades work per step against resident waves per CU.  Model: relative step
 // cost at tile 1/2/4/8 (measured on gfx950 with 1024-row partitions) 1.0,
@@ -312,7 +312,7 @@ void choose_rhs_slots(spike_factor_s *f)
     f->ncu = ncu;
     f->lds_cu = lds_cu;
     const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
-    const size_t x_per = (size_t)(pl.kmax + 2) * vb;
+    const size_t x_per = (size_t)(pl.kmax + spike::XS_PAD) * vb;
     const size_t c_per = (size_t)f->cmax * vb;
     f->cpl_fused = x_per + c_per <= lim;
     static const double step_cost[4] = {1.0, 1.22, 2.1, 4.2};
@@ -363,7 +363,7 @@ int tune_tiles(spike_factor_s *f)
     const int n = f->plan.n;
     const size_t lim = spike::block_solve_lds_limit();
     const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
-    const size_t x_per = (size_t)(f->plan.kmax + 2) * vb;
+    const size_t x_per = (size_t)(f->plan.kmax + spike::XS_PAD) * vb;
     double *db = nullptr, *dx = nullptr;
     int rc;
     if ((rc = f->dalloc(&db, (size_t)n * 8)) || (rc = f->dalloc(&dx, (size_t)n * 8))) return rc;
@@ -1475,7 +1475,7 @@ int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpe
     f->batch_slots = g.slots;
     f->vec_len = bt->off[bt->count];
     const size_t vb = precision == SPIKE_PREC_FP32_REFINED ? sizeof(float) : sizeof(double);
-    const int kcap = std::min(kmax_limit_for_device(), (int)(spike::block_solve_lds_limit() / (vb * g.slots)) - 2);
+    const int kcap = std::min(kmax_limit_for_device(), (int)(spike::block_solve_lds_limit() / (vb * g.slots)) - spike::XS_PAD);
     int64_t r0 = 0;
     ia[0] = 0;
     for (int i : g.mats) {

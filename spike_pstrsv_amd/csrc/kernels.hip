@@ -234,7 +234,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, s + i);
         }
     } else {
-        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + 2) * NR, false, roff, RS);
+        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + XS_PAD) * NR, false, roff, RS);
     }
     if (lane == 0)
 #pragma unroll
@@ -289,17 +289,39 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 for (int r = 0; r < NR; r++) xv[m][r] = xs[(unsigned)rc[d][m] * NR + r];
 #pragma unroll
             for (int r = 0; r < NR; r++) bv[r] = xs[arow * NR + r];
+            if constexpr (NR == 1) {
+                // One RHS: the accumulator is folded into lane 0 before the
+                // reduction; lane 0 of a valid slot writes the row and every
+                // other lane writes to its own spare slot k+2+lane, so the
+                // write needs no branch (measured 10% faster per step than a
+                // branch, and than the same-address form below).
+                t[0] = (T)0;
 #pragma unroll
-            for (int r = 0; r < NR; r++) {
-                t[r] = (T)0;
+                for (int m = 0; m < M; m++) t[0] = fma(-rv[d][m], xv[m][0], t[0]);
+                t[0] = li == 0 ? t[0] + bv[0] : t[0];
+                t[0] = group_sum<G>(t[0]);
+                const unsigned wrow = (li == 0 && valid) ? row : (unsigned)(k + 2 + lane);
+                xs[wrow] = t[0] * rmul[d];
+            } else {
+                // keep the accumulator reads next to the operand reads (their
+                // latency then overlaps instead of following the reduction)
 #pragma unroll
-                for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
-                t[r] = li == 0 ? t[r] + bv[r] : t[r];
-                t[r] = group_sum<G>(t[r]);
+                for (int r = 0; r < NR; r++) asm volatile("" ::"v"(bv[r]));
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    t[r] = (T)0;
+#pragma unroll
+                    for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
+                    t[r] = group_sum<G>(t[r]);
+                }
+                // Every lane of a row slot holds the same sum, accumulator and
+                // multiplier, so all of them write the same value to the row
+                // (an empty slot writes to the unused slot k+1): no branch on
+                // the step chain.
+                const unsigned wrow = valid ? row : (unsigned)(k + 1);
+#pragma unroll
+                for (int r = 0; r < NR; r++) xs[wrow * NR + r] = (bv[r] + t[r]) * rmul[d];
             }
-            if (li == 0 && valid)
-#pragma unroll
-                for (int r = 0; r < NR; r++) xs[row * NR + r] = t[r] * rmul[d];
         };
         // Refill slot d with step sidx + D (clamped to the last step), after
         // the last use of the slot's registers.
@@ -458,7 +480,7 @@ template <int G, int D, int PASS, int NR, class T>
 hipError_t launch_gdt(int P, const BlockArgs &a, hipStream_t s)
 {
     constexpr int M = 1;
-    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * NR * sizeof(T);
+    size_t lds = (size_t)(a.kmax + XS_PAD + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * NR * sizeof(T);
     auto kern = k_block_solve<G, M, D, PASS, NR, T>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

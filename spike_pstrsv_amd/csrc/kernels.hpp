@@ -6,6 +6,11 @@
 
 namespace spike {
 
+// LDS slots of a block-solve x segment after the partition's k rows: slot k
+// holds 0 (target of padding entries), slot k+1 receives writes of empty row
+// slots, slots k+2 .. k+65 receive the writes of non-leader lanes.
+constexpr int XS_PAD = 66;
+
 // Per-partition record read by the block-solve kernels.
 struct DevPart {
     int s, k;       // rows [s, s+k) of the lower-triangular form
