@@ -318,6 +318,23 @@ int spike_server_reset_stats(spike_server_t s);
 int spike_server_destroy(spike_server_t s);
 const char *spike_server_last_error(void);
 
+/* ------------------------------------------------------------------ */
+/* MatrixMarket ingest                                                   */
+/* ------------------------------------------------------------------ */
+
+/* Replaces matrixMarketToCSR, reference src/misc.c:109-245 (called from
+ * test/ex.c and test/runtime_analysis.c): reads a real/integer coordinate
+ * MatrixMarket file into zero-based CSR (columns ascending).  uplo 'L' keeps
+ * the lower triangle, 'U' the upper, 'N' everything; symm 'Y' mirrors every
+ * off-diagonal entry of a symmetric file, 'S'/'N' read entries as stored, 'A'
+ * takes 'Y' when the banner says symmetric.  *a, *ia, *ja are allocated by
+ * the library; release them with spike_free.  Errors: SPIKE_ERR_ARG (cannot
+ * open), SPIKE_ERR_MATRIX (malformed), SPIKE_ERR_UNSUPPORTED (pattern or
+ * complex files); spike_mm_last_error() describes the last error. */
+int spike_read_matrix_market(const char *path, char uplo, char symm, int *n, double **a, int **ia, int **ja);
+void spike_free(void *p);
+const char *spike_mm_last_error(void);
+
 /* Host-only symbolic analysis (no device needed): fills st with the sizes the
  * factorisation would have, except fields that need the spikes (nnz_reduced,
  * reduced_levels, dropped_rowsum_max, device_bytes are 0). */

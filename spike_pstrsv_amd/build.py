@@ -49,7 +49,7 @@ def build_product() -> Path:
           str(CSRC / "kernels.hip"), "-o", str(o)])
     objs.append(o)
     # host code
-    for src in ("plan.cpp", "api.cpp", "server.cpp"):
+    for src in ("plan.cpp", "api.cpp", "server.cpp", "mmio.cpp"):
         o = BUILD / (Path(src).stem + ".o")
         _run(["g++", "-c", "-O2", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter",
               "-D__HIP_PLATFORM_AMD__", "-I", str(ROCM / "include"), *inc, str(CSRC / src), "-o", str(o)])

@@ -34,6 +34,7 @@ EXPORTED = [
     "spike_server_options_default", "spike_server_create", "spike_server_add", "spike_server_submit",
     "spike_server_wait", "spike_server_drain", "spike_server_stats", "spike_server_reset_stats",
     "spike_server_destroy", "spike_server_last_error",
+    "spike_read_matrix_market", "spike_free", "spike_mm_last_error",
 ]
 
 
@@ -176,6 +177,13 @@ lib.spike_server_destroy.argtypes = [_vp]
 lib.spike_server_destroy.restype = _i
 lib.spike_server_last_error.argtypes = []
 lib.spike_server_last_error.restype = C.c_char_p
+lib.spike_read_matrix_market.argtypes = [C.c_char_p, _c, _c, C.POINTER(_i), C.POINTER(C.POINTER(_d)),
+                                         C.POINTER(C.POINTER(_i)), C.POINTER(C.POINTER(_i))]
+lib.spike_read_matrix_market.restype = _i
+lib.spike_free.argtypes = [_vp]
+lib.spike_free.restype = None
+lib.spike_mm_last_error.argtypes = []
+lib.spike_mm_last_error.restype = C.c_char_p
 lib.spike_pstrsv_get_timers.restype = _i
 
 
@@ -455,3 +463,26 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def read_matrix_market(path, uplo: str = "L", symm: str = "A"):
+    """CSR (a, ia, ja, n) of a MatrixMarket file (spike_read_matrix_market)."""
+    n = C.c_int(0)
+    pa = C.POINTER(_d)()
+    pia = C.POINTER(_i)()
+    pja = C.POINTER(_i)()
+    rc = lib.spike_read_matrix_market(str(path).encode(), uplo.encode()[:1], symm.encode()[:1], C.byref(n),
+                                      C.byref(pa), C.byref(pia), C.byref(pja))
+    if rc != SPIKE_OK:
+        raise SpikeError(rc, (lib.spike_mm_last_error() or b"").decode())
+    try:
+        nn = n.value
+        ia = np.ctypeslib.as_array(pia, shape=(nn + 1,)).copy()
+        nnz = int(ia[-1])
+        ja = np.ctypeslib.as_array(pja, shape=(max(nnz, 1),))[:nnz].copy()
+        a = np.ctypeslib.as_array(pa, shape=(max(nnz, 1),))[:nnz].copy()
+    finally:
+        lib.spike_free(C.cast(pa, C.c_void_p))
+        lib.spike_free(C.cast(pia, C.c_void_p))
+        lib.spike_free(C.cast(pja, C.c_void_p))
+    return a, ia, ja, nn

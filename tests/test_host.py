@@ -171,3 +171,73 @@ def test_acceptance_program_compiles_and_links(tmp_path):
                         f"-Wl,-rpath,{libdir}", "-lm", "-o", str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert exe.exists()
+
+
+def write_mm(path, n, rows, cols, vals, symmetric=False, comments=("% test matrix",)):
+    with open(path, "w") as fh:
+        fh.write("%%MatrixMarket matrix coordinate real " + ("symmetric" if symmetric else "general") + "\n")
+        for c in comments:
+            fh.write(c + "\n")
+        fh.write(f"{n} {n} {len(rows)}\n")
+        for r, c, v in zip(rows, cols, vals):
+            fh.write(f"{int(r) + 1} {int(c) + 1} {float(v)!r}\n")
+
+
+def test_matrix_market_general_and_triangles(tmp_path):
+    """Reader semantics of the reference's matrixMarketToCSR (misc.c:109-245),
+    checked against a dense matrix built from the same coordinate list."""
+    from spike_pstrsv_amd import native
+
+    rng = np.random.default_rng(5)
+    n = 40
+    D = np.zeros((n, n))
+    mask = rng.random((n, n)) < 0.15
+    D[mask] = rng.uniform(-1, 1, mask.sum())
+    np.fill_diagonal(D, 2.0 + rng.random(n))
+    rows, cols = np.nonzero(D)
+    perm = rng.permutation(len(rows))                      # file order is arbitrary
+    write_mm(tmp_path / "g.mtx", n, rows[perm], cols[perm], D[rows[perm], cols[perm]],
+             comments=("% comment one", "%  comment two"))
+    for uplo, ref in (("N", D), ("L", np.tril(D)), ("U", np.triu(D))):
+        a, ia, ja, nn = native.read_matrix_market(tmp_path / "g.mtx", uplo, "N")
+        assert nn == n
+        M = np.zeros((n, n))
+        for i in range(n):
+            cols_i = ja[ia[i]:ia[i + 1]]
+            assert np.all(np.diff(cols_i) > 0), "columns ascending"
+            M[i, cols_i] = a[ia[i]:ia[i + 1]]
+        assert np.array_equal(M, ref), uplo
+
+
+def test_matrix_market_symmetric_expansion(tmp_path):
+    from spike_pstrsv_amd import native
+
+    m = gen.banded(60, 8, 3, 0.1, 9)
+    n = m.n
+    L = np.zeros((n, n))
+    for i in range(n):
+        L[i, m.ja[m.ia[i]:m.ia[i + 1]]] = m.a[m.ia[i]:m.ia[i + 1]]
+    S = L + np.tril(L, -1).T                                # symmetric, lower triangle stored
+    r, c = np.nonzero(np.tril(S))
+    write_mm(tmp_path / "s.mtx", n, r, c, S[r, c], symmetric=True)
+    for symm in ("Y", "A"):
+        a, ia, ja, nn = native.read_matrix_market(tmp_path / "s.mtx", "U", symm)
+        U = np.zeros((n, n))
+        for i in range(n):
+            U[i, ja[ia[i]:ia[i + 1]]] = a[ia[i]:ia[i + 1]]
+        assert np.array_equal(U, np.triu(S))
+    a, ia, ja, nn = native.read_matrix_market(tmp_path / "s.mtx", "L", "Y")
+    assert np.array_equal(a, m.a) and np.array_equal(ia, m.ia) and np.array_equal(ja, m.ja)
+
+
+def test_matrix_market_errors(tmp_path):
+    from spike_pstrsv_amd import native
+
+    with pytest.raises(native.SpikeError):
+        native.read_matrix_market(tmp_path / "missing.mtx")
+    (tmp_path / "bad.mtx").write_text("%%MatrixMarket matrix coordinate real general\n3 3 2\n1 1 1.0\n")
+    with pytest.raises(native.SpikeError):
+        native.read_matrix_market(tmp_path / "bad.mtx")
+    (tmp_path / "pat.mtx").write_text("%%MatrixMarket matrix coordinate pattern general\n2 2 1\n1 1\n")
+    with pytest.raises(native.SpikeError):
+        native.read_matrix_market(tmp_path / "pat.mtx")
