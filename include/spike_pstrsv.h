@@ -166,6 +166,14 @@ int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int
 
 int spike_factor_destroy(spike_factor_t f);
 
+/* Writes a factor (partition maps, block-solve schedules, spikes' reduced
+ * system, coupling data, solver choices) to a file, so that a later process
+ * can skip preprocessing; spike_factor_load re-creates it in device memory
+ * on the current device.  The file is native-endian and tied to this
+ * library's version; batch groups are not saved. */
+int spike_factor_save(spike_factor_t f, const char *path);
+int spike_factor_load(spike_factor_t *f, const char *path);
+
 typedef struct {
     int n;
     int partitions;           /* effective P                                   */

@@ -83,6 +83,7 @@ struct spike_factor_s {
     spike_stats_t stats{};
     int device = 0;
     std::vector<void *> allocs;
+    std::vector<size_t> alloc_bytes;   // size of allocs[i]
     int64_t device_bytes = 0;
 
     DevPart *d_parts = nullptr;
@@ -161,6 +162,7 @@ struct spike_factor_s {
             return fail(SPIKE_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
         }
         allocs.push_back(*p);
+        alloc_bytes.push_back(count * sizeof(T));
         device_bytes += (int64_t)(count * sizeof(T));
         return SPIKE_OK;
     }
@@ -209,6 +211,7 @@ struct spike_factor_s {
         cap_stream = nullptr;
         for (void *p : allocs) (void)hipFree(p);
         allocs.clear();
+        alloc_bytes.clear();
         device_bytes = 0;
     }
 };
@@ -253,6 +256,7 @@ void dfree(spike_factor_s *f, void *p, size_t bytes)
 {
     auto it = std::find(f->allocs.begin(), f->allocs.end(), p);
     if (it != f->allocs.end()) {
+        f->alloc_bytes.erase(f->alloc_bytes.begin() + (it - f->allocs.begin()));
         f->allocs.erase(it);
         f->device_bytes -= (int64_t)bytes;
     }
@@ -857,6 +861,252 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.preproc_seconds = now_s() - t0;
     g_timers[0] += st.preproc_seconds;
     if (f->opts.verbose) spike_factor_log(f);
+    return SPIKE_OK;
+}
+
+// ---------------------------------------------------------------- serialization
+
+// File layout: magic, a block of scalars, then tagged records
+// {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
+// stored by content; workspaces only by size (re-allocated on load).
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '1'};
+
+struct FactorScalars {
+    spike_options_t opts;
+    spike_stats_t stats;
+    char uplo;
+    int flip, n, P, kmax, nlev, red_in_lds, sweeps, ring, cpl_fused, fuse_at[4], tile_at[4], max_nr, cmax;
+    int fp32, refine_steps, method, caller_parts, nlev_full, lev_G, nres, G, M, R, fwd;
+    int64_t nnz, spike_area;
+    double refine_rate, refine_est;
+    uint64_t lds_needed;
+};
+
+// The device arrays of a factor, by tag.
+std::vector<std::pair<const char *, void **>> device_fields(spike_factor_s *f)
+{
+    return {{"part", (void **)&f->d_parts},          {"deps", (void **)&f->d_dep_rows},
+            {"crow", (void **)&f->d_cpl_rows},       {"cptr", (void **)&f->d_cpl_ptr},
+            {"ccol", (void **)&f->d_cpl_col},        {"cval", (void **)&f->d_cpl_val},
+            {"cv32", (void **)&f->d_cpl_val32},      {"sval", (void **)&f->steps.val},
+            {"scol", (void **)&f->steps.col},        {"srow", (void **)&f->steps.row},
+            {"smul", (void **)&f->steps.mult},       {"slat", (void **)&f->steps.late},
+            {"spb_", (void **)&f->steps.part_begin}, {"sv32", (void **)&f->steps.val32},
+            {"sm32", (void **)&f->steps.mult32},     {"invd", (void **)&f->d_inv_diag},
+            {"rrp_", (void **)&f->d_rrp},            {"rci_", (void **)&f->d_rci},
+            {"rv__", (void **)&f->d_rv},             {"rlst", (void **)&f->d_rlist},
+            {"rlev", (void **)&f->d_levptr},         {"swrw", (void **)&f->d_sweep_rows},
+            {"a___", (void **)&f->d_a},              {"ia__", (void **)&f->d_ia},
+            {"ja__", (void **)&f->d_ja},             {"lrow", (void **)&f->d_lrows},
+            {"a32_", (void **)&f->d_a32},            {"id32", (void **)&f->d_invd32},
+            {"res_", (void **)&f->d_res}};
+}
+std::vector<std::pair<const char *, double **>> workspace_fields(spike_factor_s *f)
+{
+    return {{"w_y_", &f->d_y}, {"w_xa", &f->d_xa}, {"w_xb", &f->d_xb},
+            {"w_bc", &f->d_bc}, {"w_r_", &f->d_r},  {"w_dl", &f->d_dlt}};
+}
+
+size_t alloc_size(const spike_factor_s *f, const void *p)
+{
+    for (size_t i = 0; i < f->allocs.size(); i++)
+        if (f->allocs[i] == p) return f->alloc_bytes[i];
+    return 0;
+}
+
+int write_rec(FILE *fp, const char *tag, const void *data, uint64_t bytes)
+{
+    if (std::fwrite(tag, 1, 4, fp) != 4 || std::fwrite(&bytes, sizeof bytes, 1, fp) != 1 ||
+        (bytes && std::fwrite(data, 1, bytes, fp) != bytes))
+        return fail(SPIKE_ERR_ARG, "write failed");
+    return SPIKE_OK;
+}
+
+template <class T>
+int write_vec(FILE *fp, const char *tag, const std::vector<T> &v)
+{
+    return write_rec(fp, tag, v.data(), v.size() * sizeof(T));
+}
+
+int factor_save(spike_factor_s *f, const char *path)
+{
+    if (!f->mats.empty()) return fail(SPIKE_ERR_UNSUPPORTED, "batch groups are not saved");
+    const Plan &pl = f->plan;
+    FactorScalars sc{};
+    sc.opts = f->opts;
+    sc.stats = f->stats;
+    sc.uplo = pl.uplo;
+    sc.flip = pl.flip;
+    sc.n = pl.n;
+    sc.P = pl.P;
+    sc.kmax = pl.kmax;
+    sc.nlev = f->nlev;
+    sc.red_in_lds = f->red_in_lds;
+    sc.sweeps = f->sweeps;
+    sc.ring = f->ring;
+    sc.cpl_fused = f->cpl_fused;
+    for (int i = 0; i < 4; i++) {
+        sc.fuse_at[i] = f->fuse_at[i];
+        sc.tile_at[i] = f->tile_at[i];
+    }
+    sc.max_nr = f->max_nr;
+    sc.cmax = f->cmax;
+    sc.fp32 = f->fp32;
+    sc.refine_steps = f->refine_steps;
+    sc.method = f->method;
+    sc.caller_parts = f->caller_parts;
+    sc.nlev_full = f->nlev_full;
+    sc.lev_G = f->lev_G;
+    sc.nres = f->nres;
+    sc.G = f->steps.G;
+    sc.M = f->steps.M;
+    sc.R = f->steps.R;
+    sc.fwd = f->steps.fwd;
+    sc.nnz = pl.nnz;
+    sc.spike_area = pl.spike_area;
+    sc.refine_rate = f->refine_rate;
+    sc.refine_est = f->refine_est;
+    {
+        const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
+        int maxt = 1;
+        for (int i = 0; i < 4; i++) maxt = std::max(maxt, f->tile_at[i]);
+        sc.lds_needed = f->method == SPIKE_METHOD_LEVELS ? 0 : (uint64_t)(pl.kmax + spike::XS_PAD) * vb * maxt;
+    }
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(SPIKE_ERR_ARG, std::string("cannot open ") + path + " for writing");
+    int rc = SPIKE_OK;
+    if (std::fwrite(kMagic, 1, 8, fp) != 8 || std::fwrite(&sc, sizeof sc, 1, fp) != 1) rc = fail(SPIKE_ERR_ARG, "write failed");
+    if (!rc) rc = write_vec(fp, "hprt", pl.parts);
+    if (!rc) rc = write_vec(fp, "hdep", pl.dep_rows);
+    if (!rc) rc = write_vec(fp, "hcpl", pl.cpl_rows);
+    if (!rc) rc = write_vec(fp, "hsta", pl.starts);
+    if (!rc) rc = write_vec(fp, "hswo", f->sweep_off);
+    if (!rc) rc = write_vec(fp, "hlvp", f->lev_ptr);
+    std::vector<char> buf;
+    for (auto &fld : device_fields(f)) {
+        if (rc) break;
+        const void *p = *fld.second;
+        if (!p) continue;
+        const size_t bytes = alloc_size(f, p);
+        if (!bytes) {
+            rc = fail(SPIKE_ERR_STATE, std::string("unknown size of device array ") + fld.first);
+            break;
+        }
+        buf.resize(bytes);
+        if (hipMemcpy(buf.data(), p, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = fail(SPIKE_ERR_DEVICE, "device read failed");
+            break;
+        }
+        rc = write_rec(fp, fld.first, buf.data(), bytes);
+    }
+    for (auto &fld : workspace_fields(f)) {
+        if (rc) break;
+        const uint64_t bytes = *fld.second ? alloc_size(f, *fld.second) : 0;
+        if (bytes) rc = write_rec(fp, fld.first, &bytes, sizeof bytes);
+    }
+    if (!rc) rc = write_rec(fp, "end_", nullptr, 0);
+    if (std::fclose(fp) != 0 && !rc) rc = fail(SPIKE_ERR_ARG, "close failed");
+    return rc;
+}
+
+int factor_load(spike_factor_s *f, const char *path)
+{
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return fail(SPIKE_ERR_ARG, std::string("cannot open ") + path);
+    struct Closer {
+        FILE *fp;
+        ~Closer() { std::fclose(fp); }
+    } closer{fp};
+    char mg[8];
+    FactorScalars sc{};
+    if (std::fread(mg, 1, 8, fp) != 8 || std::memcmp(mg, kMagic, 8) != 0)
+        return fail(SPIKE_ERR_ARG, "not a saved spike_pstrsv factor (bad magic)");
+    if (std::fread(&sc, sizeof sc, 1, fp) != 1) return fail(SPIKE_ERR_ARG, "truncated factor file");
+    if (sc.lds_needed > spike::block_solve_lds_limit())
+        return fail(SPIKE_ERR_UNSUPPORTED, "factor needs more LDS per workgroup than this device has");
+    HIP_TRY(hipGetDevice(&f->device));
+    Plan &pl = f->plan;
+    f->opts = sc.opts;
+    f->stats = sc.stats;
+    pl.uplo = sc.uplo;
+    pl.flip = sc.flip != 0;
+    pl.n = sc.n;
+    pl.P = sc.P;
+    pl.kmax = sc.kmax;
+    pl.nnz = sc.nnz;
+    pl.spike_area = sc.spike_area;
+    f->nlev = sc.nlev;
+    f->red_in_lds = sc.red_in_lds != 0;
+    f->sweeps = sc.sweeps;
+    f->ring = sc.ring;
+    f->cpl_fused = sc.cpl_fused != 0;
+    for (int i = 0; i < 4; i++) {
+        f->fuse_at[i] = sc.fuse_at[i] != 0;
+        f->tile_at[i] = sc.tile_at[i];
+    }
+    f->max_nr = sc.max_nr;
+    f->cmax = sc.cmax;
+    f->fp32 = sc.fp32 != 0;
+    f->refine_steps = sc.refine_steps;
+    f->refine_rate = sc.refine_rate;
+    f->refine_est = sc.refine_est;
+    f->method = sc.method;
+    f->caller_parts = sc.caller_parts != 0;
+    f->nlev_full = sc.nlev_full;
+    f->lev_G = sc.lev_G;
+    f->nres = sc.nres;
+    f->steps.G = sc.G;
+    f->steps.M = sc.M;
+    f->steps.R = sc.R;
+    f->steps.fwd = sc.fwd;
+    (void)hipDeviceGetAttribute(&f->ncu, hipDeviceAttributeMultiprocessorCount, f->device);
+    (void)hipDeviceGetAttribute(&f->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, f->device);
+    (void)hipGetLastError();
+    auto dev = device_fields(f);
+    auto ws = workspace_fields(f);
+    for (;;) {
+        char tag[5] = {0};
+        uint64_t bytes = 0;
+        if (std::fread(tag, 1, 4, fp) != 4 || std::fread(&bytes, sizeof bytes, 1, fp) != 1)
+            return fail(SPIKE_ERR_ARG, "truncated factor file");
+        if (std::strcmp(tag, "end_") == 0) break;
+        if (bytes > ((uint64_t)1 << 40)) return fail(SPIKE_ERR_ARG, "corrupt factor file");
+        std::vector<char> buf(bytes);
+        if (bytes && std::fread(buf.data(), 1, bytes, fp) != bytes) return fail(SPIKE_ERR_ARG, "truncated factor file");
+        auto host = [&](auto &vec) {
+            using T = typename std::decay_t<decltype(vec)>::value_type;
+            vec.resize(bytes / sizeof(T));
+            if (bytes) std::memcpy(vec.data(), buf.data(), bytes);
+        };
+        if (!std::strcmp(tag, "hprt")) { host(pl.parts); continue; }
+        if (!std::strcmp(tag, "hdep")) { host(pl.dep_rows); continue; }
+        if (!std::strcmp(tag, "hcpl")) { host(pl.cpl_rows); continue; }
+        if (!std::strcmp(tag, "hsta")) { host(pl.starts); continue; }
+        if (!std::strcmp(tag, "hswo")) { host(f->sweep_off); continue; }
+        if (!std::strcmp(tag, "hlvp")) { host(f->lev_ptr); continue; }
+        bool done = false;
+        for (auto &fld : dev)
+            if (!std::strcmp(tag, fld.first)) {
+                char *p = nullptr;
+                int rc = f->dalloc(&p, (size_t)bytes);
+                if (rc) return rc;
+                HIP_TRY(hipMemcpy(p, buf.data(), bytes, hipMemcpyHostToDevice));
+                *fld.second = p;
+                done = true;
+                break;
+            }
+        for (auto &fld : ws)
+            if (!done && !std::strcmp(tag, fld.first)) {
+                uint64_t wb = 0;
+                std::memcpy(&wb, buf.data(), sizeof wb);
+                int rc = f->dalloc(fld.second, (size_t)(wb / sizeof(double)));
+                if (rc) return rc;
+                done = true;
+                break;
+            }
+        if (!done) return fail(SPIKE_ERR_ARG, std::string("unknown record ") + tag);
+    }
+    f->stats.device_bytes = f->device_bytes;
     return SPIKE_OK;
 }
 
@@ -1899,6 +2149,34 @@ int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int
     if (!f) return fail(SPIKE_ERR_STATE, "factor is NULL");
     if (!phase_ms) return fail(SPIKE_ERR_ARG, "phase_ms is NULL");
     return factor_solve(f, nrhs, b, ldb, x, ldx, (hipStream_t)stream, true, phase_ms);
+}
+
+int spike_factor_save(spike_factor_t f, const char *path)
+{
+    if (!f || !path) return fail(SPIKE_ERR_ARG, "NULL argument");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != f->device) HIP_TRY(hipSetDevice(f->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return factor_save(f, path);
+}
+
+int spike_factor_load(spike_factor_t *out, const char *path)
+{
+    if (!out || !path) return fail(SPIKE_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    spike_factor_s *f = new spike_factor_s();
+    int rc = factor_load(f, path);
+    if (rc != SPIKE_OK) {
+        std::string keep = t_err;
+        f->release();
+        delete f;
+        t_err = keep;
+        return rc;
+    }
+    *out = f;
+    t_err.clear();
+    return SPIKE_OK;
 }
 
 int spike_factor_destroy(spike_factor_t f)

@@ -35,6 +35,7 @@ EXPORTED = [
     "spike_server_wait", "spike_server_drain", "spike_server_stats", "spike_server_reset_stats",
     "spike_server_destroy", "spike_server_last_error",
     "spike_read_matrix_market", "spike_free", "spike_mm_last_error",
+    "spike_factor_save", "spike_factor_load",
 ]
 
 
@@ -180,6 +181,10 @@ lib.spike_server_last_error.restype = C.c_char_p
 lib.spike_read_matrix_market.argtypes = [C.c_char_p, _c, _c, C.POINTER(_i), C.POINTER(C.POINTER(_d)),
                                          C.POINTER(C.POINTER(_i)), C.POINTER(C.POINTER(_i))]
 lib.spike_read_matrix_market.restype = _i
+lib.spike_factor_save.argtypes = [_vp, C.c_char_p]
+lib.spike_factor_save.restype = _i
+lib.spike_factor_load.argtypes = [C.POINTER(_vp), C.c_char_p]
+lib.spike_factor_load.restype = _i
 lib.spike_free.argtypes = [_vp]
 lib.spike_free.restype = None
 lib.spike_mm_last_error.argtypes = []
@@ -276,6 +281,20 @@ class Factor:
         _check(lib.spike_factor_solve_profiled(self.handle, nrhs, _ptr(b), n if ldb is None else ldb, _ptr(x),
                                                n if ldx is None else ldx, stream or None, ms))
         return list(ms)
+
+    def save(self, path) -> None:
+        _check(lib.spike_factor_save(self.handle, str(path).encode()))
+
+    @classmethod
+    def load(cls, path) -> "Factor":
+        h = _vp()
+        _check(lib.spike_factor_load(C.byref(h), str(path).encode()))
+        f = cls.__new__(cls)
+        f.handle = h.value
+        st = Stats()
+        _check(lib.spike_factor_stats(f.handle, C.byref(st)))
+        f.n = st.n
+        return f
 
     def stats(self) -> dict:
         st = Stats()

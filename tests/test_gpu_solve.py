@@ -222,3 +222,34 @@ def test_acceptance_program(tmp_path):
     print(run.stdout[-3000:])
     assert run.returncode == 0 and "PASSED" in run.stdout, run.stdout + run.stderr
     assert "PSTRSV" in run.stdout or "partition" in run.stdout.lower()
+
+
+@pytest.mark.parametrize("case", ["banded_L", "banded_U_fp32", "lap2d_exact", "lap3d_levels"])
+def test_factor_save_load(native, tmp_path, case):
+    """A saved and reloaded factor solves bit for bit like the original."""
+    if case == "banded_L":
+        m, kw = gen.banded(1 << 15, seed=gen.SEED_BASE + 90), {"partitions": 32}
+        uplo = "L"
+    elif case == "banded_U_fp32":
+        m, kw = gen.to_upper(gen.banded(1 << 15, seed=gen.SEED_BASE + 91)), {"precision": "fp32_refined"}
+        uplo = "U"
+    elif case == "lap2d_exact":
+        m, kw = gen.lap2d(128), {"partitions": 8, "reduced_solver": "exact"}
+        uplo = "L"
+    else:
+        m, kw = gen.lap3d(40), {"method": "levels"}
+        uplo = "L"
+    B = gen.rhs(m.n, 3, seed=gen.SEED_BASE + 92)
+    f = native.Factor(uplo, m.a, m.ia, m.ja, **kw)
+    X0 = f.solve(B, nrhs=3)
+    path = tmp_path / "factor.spk"
+    f.save(path)
+    s0 = f.stats()
+    f.close()
+    g = native.Factor.load(path)
+    s1 = g.stats()
+    for k in ("n", "partitions", "n_reduced", "method", "refine_steps", "rhs_tile", "reduced_sweeps"):
+        assert s0[k] == s1[k], k
+    X1 = g.solve(B, nrhs=3)
+    assert np.array_equal(X0, X1)
+    assert relerr(X1[0], orc.trsv(uplo, m.a, m.ia, m.ja, B[0])) <= TOL

@@ -241,3 +241,14 @@ def test_matrix_market_errors(tmp_path):
     (tmp_path / "pat.mtx").write_text("%%MatrixMarket matrix coordinate pattern general\n2 2 1\n1 1\n")
     with pytest.raises(native.SpikeError):
         native.read_matrix_market(tmp_path / "pat.mtx")
+
+
+def test_factor_load_rejects_foreign_file(tmp_path):
+    from spike_pstrsv_amd import native
+
+    p = tmp_path / "x.spk"
+    p.write_bytes(b"NOTAFACTORFILE" * 10)
+    with pytest.raises(native.SpikeError):
+        native.Factor.load(p)
+    with pytest.raises(native.SpikeError):
+        native.Factor.load(tmp_path / "missing.spk")
