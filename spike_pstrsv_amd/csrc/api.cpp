@@ -236,14 +236,10 @@ int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps
         return rc;
     if (fp32) {
         if (ss.fwd) return fail(SPIKE_ERR_UNSUPPORTED, "the forwarding block solve has no FP32 form");
-        if ((rc = f->upload(&val32, to_f32(ss.val))) || (rc = f->upload(&mult32, to_f32(ss.mult)))) return rc;
+        if ((rc = f->upload(&val32, to_f32(ss.val)))) return rc;
     } else {
         if ((rc = f->upload(&val, ss.val))) return rc;
-        if (ss.fwd) {
-            if ((rc = f->upload(&late, ss.late))) return rc;
-        } else if ((rc = f->upload(&mult, ss.mult))) {
-            return rc;
-        }
+        if (ss.fwd && (rc = f->upload(&late, ss.late))) return rc;
     }
     ds.val = val; ds.col = col; ds.row = row; ds.mult = mult; ds.late = late; ds.part_begin = pb;
     ds.val32 = val32; ds.mult32 = mult32;

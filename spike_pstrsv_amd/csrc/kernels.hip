@@ -2,12 +2,12 @@
 // solves for the iteratively refined mode).
 //
 // Block solve (pass 1 / pass 2): one 64-lane wavefront per partition.  The
-// partition's x segment lives in LDS (k+1 doubles; slot k holds 0.0 and is the
-// target of padding entries).  Before the steps run, xs[r] holds the row's
-// right-hand side; a step slot then computes
-//     xs[row] = (xs[row] - sum_e val_e * xs[col_e]) * mult
-// where mult = 1/diag for a row's final chunk and 1 for a partial chunk of a
-// long row.  Steps execute in order and each step only reads rows written by
+// partition's x segment lives in LDS (k + XS_PAD slots; slot k holds 0.0 and
+// is the target of padding entries).  Step values are pre-scaled by their
+// row's 1/diag; before the steps run, xs[r] holds the row's right-hand side
+// times 1/diag, and a step slot computes
+//     xs[row] = xs[row] - sum_e val_e * xs[col_e]
+// (the chunks of a long row subtract their partial sums in turn).  Steps execute in order and each step only reads rows written by
 // earlier steps; LDS operations of one wavefront complete in program order, so
 // no flags or barriers are needed between steps.  Step records are streamed
 // from HBM through a register ring D steps deep, so the latency of the loads
@@ -97,7 +97,6 @@ __device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff
 template <class T> struct VT;
 template <> struct VT<double> {
     static __device__ __forceinline__ const double *val(const BlockArgs &A) { return A.st.val; }
-    static __device__ __forceinline__ const double *mult(const BlockArgs &A) { return A.st.mult; }
     static __device__ __forceinline__ const double *cpl(const BlockArgs &A) { return A.cpl_val; }
     static __device__ __forceinline__ double load(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s)
     {
@@ -106,7 +105,6 @@ template <> struct VT<double> {
 };
 template <> struct VT<float> {
     static __device__ __forceinline__ const float *val(const BlockArgs &A) { return A.st.val32; }
-    static __device__ __forceinline__ const float *mult(const BlockArgs &A) { return A.st.mult32; }
     static __device__ __forceinline__ const float *cpl(const BlockArgs &A) { return A.cpl_val32; }
     static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s)
     {
@@ -230,11 +228,12 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
         for (int i = lane; i < k; i += 64) {
+            const T sc = (T)A.inv_diag[s + i];
 #pragma unroll
-            for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, s + i);
+            for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, s + i) * sc;
         }
     } else {
-        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + XS_PAD) * NR, false, roff, RS);
+        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + XS_PAD) * NR, true, roff, RS);
     }
     if (lane == 0)
 #pragma unroll
@@ -248,7 +247,6 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const __amdgpu_buffer_rsrc_t bval = make_rsrc(VT<T>::val(A) + (size_t)sb * M * 64, nstu * M * 64u * TB);
     const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
     const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
-    const __amdgpu_buffer_rsrc_t bmul = make_rsrc(VT<T>::mult(A) + (size_t)sb * R, nstu * R * TB);
     // Register ring of D step records.  It starts as empty padding steps
     // (column k, no row) and the loop starts D steps early, so the first D
     // iterations compute nothing and only issue loads.  Every load then comes
@@ -261,7 +259,6 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     T rv[D][M];
     uint16_t rc[D][M];
     uint16_t rrow[D];
-    T rmul[D];
     if (nst <= 0) goto epilogue;
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -271,7 +268,6 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             rc[d][m] = (uint16_t)k;
         }
         rrow[d] = (uint16_t)0xFFFFu;
-        rmul[d] = (T)0;
     }
     {
         // One step from ring slot d: LDS reads first (operands and the row's
@@ -301,7 +297,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 t[0] = li == 0 ? t[0] + bv[0] : t[0];
                 t[0] = group_sum<G>(t[0]);
                 const unsigned wrow = (li == 0 && valid) ? row : (unsigned)(k + 2 + lane);
-                xs[wrow] = t[0] * rmul[d];
+                xs[wrow] = t[0];
             } else {
                 // keep the accumulator reads next to the operand reads (their
                 // latency then overlaps instead of following the reduction)
@@ -314,13 +310,13 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                     for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
                     t[r] = group_sum<G>(t[r]);
                 }
-                // Every lane of a row slot holds the same sum, accumulator and
-                // multiplier, so all of them write the same value to the row
+                // Every lane of a row slot holds the same sum and accumulator,
+                // so all of them write the same value to the row
                 // (an empty slot writes to the unused slot k+1): no branch on
                 // the step chain.
                 const unsigned wrow = valid ? row : (unsigned)(k + 1);
 #pragma unroll
-                for (int r = 0; r < NR; r++) xs[wrow * NR + r] = (bv[r] + t[r]) * rmul[d];
+                for (int r = 0; r < NR; r++) xs[wrow * NR + r] = bv[r] + t[r];
             }
         };
         // Refill slot d with step sidx + D (clamped to the last step), after
@@ -333,7 +329,6 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 rc[d][m] = buf_u16(bcol, (unsigned)(m * 64 + lane) * 2u, nxt * (unsigned)(M * 128));
             }
             rrow[d] = buf_u16(brow, (unsigned)slot * 2u, nxt * (unsigned)(R * 2));
-            rmul[d] = VT<T>::load(bmul, (unsigned)slot * TB, nxt * (unsigned)R * TB);
         };
         int base = -D;
         for (; base + D <= nst; base += D) {

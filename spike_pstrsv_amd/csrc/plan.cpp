@@ -135,18 +135,20 @@ void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
     val.assign((size_t)nsteps * M * 64, 0.0);
     col.assign((size_t)nsteps * M * 64, (uint16_t)k);
     if (!fwd) {
+        // values pre-scaled by the row's 1/diag (the right-hand side is
+        // scaled on load), so every chunk of a row computes x -= sum val * x
         row.assign((size_t)nsteps * R, (uint16_t)0xFFFF);
-        mult.assign((size_t)nsteps * R, 0.0);
+        mult.clear();
         for (int s = 0; s < nsteps; s++) {
             for (int slot = 0; slot < (int)steps[s].size(); slot++) {
                 const Task &t = tasks[steps[s][slot]];
+                const double inv = 1.0 / diag[t.row];
                 row[(size_t)s * R + slot] = (uint16_t)t.row;
-                mult[(size_t)s * R + slot] = t.final_chunk ? 1.0 / diag[t.row] : 1.0;
                 for (int e = t.eb; e < t.ee; e++) {
                     int q = e - t.eb;
                     int m = q / G, li = q % G;
                     size_t idx = ((size_t)s * M + m) * 64 + slot * G + li;
-                    val[idx] = evals[e];
+                    val[idx] = evals[e] * inv;
                     col[idx] = (uint16_t)ecol[e];
                 }
             }
@@ -403,14 +405,13 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     ss.col.resize((size_t)ss.nsteps * M * 64);
     ss.row.resize((size_t)ss.nsteps * ss.R);
     if (fwd) ss.late.resize((size_t)ss.nsteps * ss.R * ss.R);
-    else ss.mult.resize((size_t)ss.nsteps * ss.R);
+    else ss.mult.clear();
     parallel_for(P, [&](int t) {
         size_t s0 = (size_t)ss.part_begin[t];
         std::copy(pv[t].begin(), pv[t].end(), ss.val.begin() + s0 * M * 64);
         std::copy(pc[t].begin(), pc[t].end(), ss.col.begin() + s0 * M * 64);
         std::copy(pr[t].begin(), pr[t].end(), ss.row.begin() + s0 * ss.R);
         if (fwd) std::copy(pl_late[t].begin(), pl_late[t].end(), ss.late.begin() + s0 * ss.R * ss.R);
-        else std::copy(pm[t].begin(), pm[t].end(), ss.mult.begin() + s0 * ss.R);
     });
     return "";
 }

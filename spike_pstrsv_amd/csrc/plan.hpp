@@ -26,9 +26,10 @@ namespace spike {
 // holds R = 64/G row slots; slot t is served by lanes [t*G, (t+1)*G) and each
 // lane holds M entries ("rounds").
 // Within a partition, steps run in order and every row (chunk) in a step only
-// depends on rows finished in earlier steps.  Rows longer than G*M entries are
-// split into chunks; a non-final chunk subtracts its partial dot product from
-// the row's accumulator and only the final chunk scales by 1/diag.
+// depends on rows finished in earlier steps.  Every value is pre-scaled by
+// its row's 1/diag and the right-hand side is scaled on load, so a step slot
+// computes x[row] -= sum val * x[col]; rows longer than G*M entries are split
+// into chunks that subtract their partial sums in turn.
 //
 // Forwarding layout (fwd = true, G = 16, R = 4): every value is pre-scaled by
 // the row's 1/diag (the right-hand side is scaled on load), so a final chunk
@@ -47,7 +48,7 @@ struct StepStream {
     std::vector<double> val;            // nsteps * M * 64  [step][round][lane]
     std::vector<uint16_t> col;          // nsteps * M * 64  local column, pad = k_p
     std::vector<uint16_t> row;          // nsteps * R       local row, 0xFFFF = empty
-    std::vector<double> mult;           // nsteps * R       1/diag (final) or 1 (chunk); not fwd
+    std::vector<double> mult;           // unused (values are pre-scaled)
     std::vector<double> late;           // nsteps * R * R   fwd only
 };
 
