@@ -98,7 +98,7 @@ struct spike_factor_s {
     int sweeps = 0;             // > 0: Jacobi sweeps, 0: exact level solve
     std::vector<int> sweep_off; // sweep m: int4 row records off[m-1] .. off[m]
     int *d_sweep_rows = nullptr;
-    int ring = 8;               // block-solve step records in flight
+    int ring = 16;              // block-solve step records in flight (tiles 1, 2; 8 above)
     bool cpl_fused = false;     // pass 2 can compute its coupling update itself
     bool fuse_at[4] = {};       // ... and does so at 1, 2, 4, 8 RHS slots
     int tile_at[4] = {1, 1, 1, 1};  // RHS per wave at 1, 2, 4, 8 slots

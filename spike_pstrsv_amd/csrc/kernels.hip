@@ -60,12 +60,27 @@ __device__ __forceinline__ void buf_2f64(__amdgpu_buffer_rsrc_t r, unsigned voff
     b = __builtin_bit_cast(double, hi);
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int l)
+// Every lane receives the values held by the four 16-lane rows of v (v is
+// uniform within each row): x[t] = v of row t.  gfx950's permlane16 swap
+// exchanges odd and even rows, the permlane32 swap the two halves; three
+// swaps per 32-bit half, all in VGPRs.
+__device__ __forceinline__ void rows4_f64(double v, double x[4])
 {
-    unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    int lo = __builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffull), l);
-    int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)(u & 0xffffffffull), hi = (unsigned)(u >> 32);
+    const auto alo = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);   // rows {0,0,2,2}, {1,1,3,3}
+    const auto ahi = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const auto blo = __builtin_amdgcn_permlane32_swap(alo[0], alo[0], false, false);   // row 0, row 2
+    const auto bhi = __builtin_amdgcn_permlane32_swap(ahi[0], ahi[0], false, false);
+    const auto clo = __builtin_amdgcn_permlane32_swap(alo[1], alo[1], false, false);   // row 1, row 3
+    const auto chi = __builtin_amdgcn_permlane32_swap(ahi[1], ahi[1], false, false);
+    auto mk = [](unsigned l, unsigned h) {
+        return __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
+    };
+    x[0] = mk(blo[0], bhi[0]);
+    x[1] = mk(clo[0], chi[0]);
+    x[2] = mk(blo[1], bhi[1]);
+    x[3] = mk(clo[1], chi[1]);
 }
 
 template <int CTRL>
@@ -365,10 +380,11 @@ epilogue:
 // are pre-scaled by 1/diag; xs[r] starts as b_r/d_r.  Step s of a partition:
 //   E_s  = xs[row] - sum_early val * xs[col]   (lanes of the slot, DPP sum)
 //   x_s  = E_s - sum_t late[s][slot][t] * x_{s-1}[t]
-// where x_{s-1}[t] is read from the previous step's registers (readlane of
-// lane 16 t).  E_{s+1} only depends on steps <= s-1, so its LDS reads and its
-// reduction are issued during step s and stay off the step-to-step chain,
-// which is readlane + 4 FMAs.  Rows of empty slots are k+1 (scratch).
+// where x_{s-1}[t] comes from the previous step's registers (row t of the
+// wave, gathered by permlane swaps).  E_{s+1} only depends on steps <= s-1,
+// so its LDS reads and its reduction are issued during step s and stay off
+// the step-to-step chain, which is two swap levels + the late FMAs.  Rows of
+// empty slots are k+1; non-leader lanes write to their own spare slot.
 template <int D, int PASS>
 __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
 {
@@ -390,7 +406,7 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
             xs[r] = m.bv(0, g) * invd[g];
         }
     } else {
-        pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + 2, true, 0, 1);
+        pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + XS_PAD, true, 0, 1);
     }
     if (lane == 0) {
         xs[k] = 0.0;
@@ -432,13 +448,11 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
         // late part of the step in ring slot d, given its early sum
         auto finish = [&](int d, double e) __attribute__((always_inline)) {
             double xp[R];
-#pragma unroll
-            for (int t = 0; t < R; t++) xp[t] = readlane_f64(xprev, t * G);
-            double l = rl[d][0] * xp[0];
-#pragma unroll
-            for (int t = 1; t < R; t++) l = fma(rl[d][t], xp[t], l);
+            rows4_f64(xprev, xp);
+            const double l = fma(rl[d][1], xp[1], rl[d][0] * xp[0]) + fma(rl[d][3], xp[3], rl[d][2] * xp[2]);
             const double x = e - l;
-            if (li == 0) xs[rrow[d]] = x;
+            const unsigned wr = li == 0 ? (unsigned)rrow[d] : (unsigned)(k + 2 + lane);
+            xs[wr] = x;
             xprev = x;
         };
         // One step: issue the LDS reads of the next step's early part, run
@@ -496,8 +510,8 @@ hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
 {
     if (a.tile < 1 || a.tile > a.nr_slots || a.nr_slots % a.tile) return hipErrorInvalidValue;
     switch (a.tile) {
-    case 1: return launch_gd<G, 8, PASS, 1>(P, a, s);
-    case 2: return launch_gd<G, 8, PASS, 2>(P, a, s);
+    case 1: return a.ring >= 16 ? launch_gd<G, 16, PASS, 1>(P, a, s) : launch_gd<G, 8, PASS, 1>(P, a, s);
+    case 2: return a.ring >= 16 ? launch_gd<G, 16, PASS, 2>(P, a, s) : launch_gd<G, 8, PASS, 2>(P, a, s);
     case 4: return launch_gd<G, 8, PASS, 4>(P, a, s);
     case 8: return launch_gd<G, 8, PASS, 8>(P, a, s);
     default: return hipErrorInvalidValue;
@@ -507,7 +521,7 @@ hipError_t launch_g(int P, const BlockArgs &a, hipStream_t s)
 template <int D, int PASS>
 hipError_t launch_fwd(int P, const BlockArgs &a, hipStream_t s)
 {
-    size_t lds = (size_t)(a.kmax + 2 + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * sizeof(double);
+    size_t lds = (size_t)(a.kmax + XS_PAD + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * sizeof(double);
     auto kern = k_block_solve_fwd<D, PASS>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
