@@ -760,14 +760,17 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
             beyond = mx[J - 1] < 1.0 ? mx[J - 1] / (1.0 - mx[J - 1]) * partial : 1e300;
             mass += beyond;
         }
-        // smallest S with sum_{j>S} mx_j <= 1e-16
+        // smallest S with sum_{j>S} mx_j <= kTol: the truncation of the
+        // reduced system already allows 1e-15 (row sum of dropped |w|), so
+        // iterating further below that gains nothing
+        constexpr double kTol = 1e-15;
         double tail = beyond;
-        if (tail <= 1e-16) {
+        if (tail <= kTol) {
             sweeps = J;
             jbound = tail;
             for (int S = J - 1; S >= 0; S--) {
                 tail += mx[S];           // mx[S] is the term j = S+1
-                if (tail > 1e-16) break;
+                if (tail > kTol) break;
                 sweeps = S;
                 jbound = tail;
             }
@@ -777,7 +780,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
                 int m = J;
                 for (int S = J - 1; S >= 0; S--) {
                     t += T[S][q];
-                    if (t > 1e-16) break;
+                    if (t > kTol) break;
                     m = S;
                 }
                 row_sweeps[q] = std::min(m, sweeps);
@@ -865,7 +868,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 // File layout: magic, a block of scalars, then tagged records
 // {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
 // stored by content; workspaces only by size (re-allocated on load).
-constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '1'};
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '2'};
 
 struct FactorScalars {
     spike_options_t opts;

@@ -282,38 +282,34 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             rv[d][m] = (T)0;
             rc[d][m] = (uint16_t)k;
         }
-        rrow[d] = (uint16_t)0xFFFFu;
+        rrow[d] = (uint16_t)(k + 1);
     }
     {
         // One step from ring slot d: LDS reads first (operands and the row's
-        // accumulator), then the dot products with the accumulator folded into
-        // lane 0 of the group, so the accumulator read stays ahead of the
-        // reduction.
+        // accumulator), then the dot products.  Empty row slots name slot
+        // k+1, so no lane tests for them.
         auto step = [&](int d) __attribute__((always_inline)) {
             const unsigned row = rrow[d];
-            const bool valid = row != 0xFFFFu;
-            const unsigned arow = valid ? row : (unsigned)k;
-            T xv[M][NR], bv[NR], t[NR];
+            T xv[M][NR], t[NR];
 #pragma unroll
             for (int m = 0; m < M; m++)
 #pragma unroll
                 for (int r = 0; r < NR; r++) xv[m][r] = xs[(unsigned)rc[d][m] * NR + r];
-#pragma unroll
-            for (int r = 0; r < NR; r++) bv[r] = xs[arow * NR + r];
             if constexpr (NR == 1) {
-                // One RHS: the accumulator is folded into lane 0 before the
-                // reduction; lane 0 of a valid slot writes the row and every
-                // other lane writes to its own spare slot k+2+lane, so the
-                // write needs no branch (measured 10% faster per step than a
-                // branch, and than the same-address form below).
-                t[0] = (T)0;
+                // One RHS: lane 0 of the group starts its dot product from
+                // the row's accumulator, the other lanes from the zero slot k
+                // (the choice is an address, off the step chain); lane 0
+                // writes the row and every other lane its own spare slot
+                // k+2+lane, so the write needs no branch either.
+                t[0] = xs[li == 0 ? row : (unsigned)k];
 #pragma unroll
                 for (int m = 0; m < M; m++) t[0] = fma(-rv[d][m], xv[m][0], t[0]);
-                t[0] = li == 0 ? t[0] + bv[0] : t[0];
                 t[0] = group_sum<G>(t[0]);
-                const unsigned wrow = (li == 0 && valid) ? row : (unsigned)(k + 2 + lane);
-                xs[wrow] = t[0];
+                xs[li == 0 ? row : (unsigned)(k + 2 + lane)] = t[0];
             } else {
+                T bv[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++) bv[r] = xs[row * NR + r];
                 // keep the accumulator reads next to the operand reads (their
                 // latency then overlaps instead of following the reduction)
 #pragma unroll
@@ -326,12 +322,10 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                     t[r] = group_sum<G>(t[r]);
                 }
                 // Every lane of a row slot holds the same sum and accumulator,
-                // so all of them write the same value to the row
-                // (an empty slot writes to the unused slot k+1): no branch on
-                // the step chain.
-                const unsigned wrow = valid ? row : (unsigned)(k + 1);
+                // so all of them write the same value to the row (slot k+1
+                // for an empty slot): no branch on the step chain.
 #pragma unroll
-                for (int r = 0; r < NR; r++) xs[wrow * NR + r] = bv[r] + t[r];
+                for (int r = 0; r < NR; r++) xs[row * NR + r] = bv[r] + t[r];
             }
         };
         // Refill slot d with step sidx + D (clamped to the last step), after
@@ -593,14 +587,15 @@ __global__ __launch_bounds__(1024) void k_reduced_solve(int nred, int nlev, cons
 // fixed summation order.  Rows of the list flagged with the top bit only copy
 // xin[r] to xout[r] (rows that converged in the previous sweep, so that both
 // buffers hold their final value).  Each lane issues all of its column and
-// value loads, then all of its gathers, before the FMAs.
+// value loads, then all of its gathers, before the FMAs; each lane sums its
+// entries in ascending order, then the group sum runs in a fixed order.
 template <int G, int U, int NR>
 __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__restrict__ rows,
                                                       const int *__restrict__ ci, const double *__restrict__ v,
                                                       const double *__restrict__ y, const double *__restrict__ xin,
                                                       double *__restrict__ xout)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int q = tid / G, li = tid % G;
     int r = -1;
     bool copy = false;
@@ -610,19 +605,28 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
         copy = dsc.x < 0;
         r = dsc.x & 0x7fffffff;
     }
+    // y[r] (xin[r] for a copy row), loaded with the entries
+    double base[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) base[t] = 0.0;
+    if (li == 0 && r >= 0)
+#pragma unroll
+        for (int t = 0; t < NR; t++) base[t] = copy ? xin[(size_t)r * NR + t] : y[(size_t)r * NR + t];
     double acc[NR];
 #pragma unroll
     for (int t = 0; t < NR; t++) acc[t] = 0.0;
     if (r >= 0 && !copy) {
+        // U entries per lane per trip, the last trip predicated, so a row of
+        // up to U * G entries issues all of its loads at once
         const int e = dsc.z;
-        int j = dsc.y + li;
-        for (; j + (U - 1) * G < e; j += U * G) {
+        for (int j = dsc.y + li; j < e; j += U * G) {
             int c[U];
             double w[U], xv[U][NR];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                c[u] = ci[j + u * G];
-                w[u] = v[j + u * G];
+                const bool ok = j + u * G < e;
+                c[u] = ok ? ci[j + u * G] : r;     // masked: the row's own (valid) entry, not used
+                w[u] = ok ? v[j + u * G] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -631,21 +635,15 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
 #pragma unroll
             for (int u = 0; u < U; u++)
 #pragma unroll
-                for (int t = 0; t < NR; t++) acc[t] = fma(w[u], xv[u][t], acc[t]);
-        }
-        for (; j < e; j += G) {
-            const double w = v[j];
-            const size_t c = (size_t)ci[j] * NR;
-#pragma unroll
-            for (int t = 0; t < NR; t++) acc[t] = fma(w, xin[c + t], acc[t]);
+                for (int t = 0; t < NR; t++)
+                    if (j + u * G < e) acc[t] = fma(w[u], xv[u][t], acc[t]);
         }
     }
 #pragma unroll
     for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
     if (li == 0 && r >= 0)
 #pragma unroll
-        for (int t = 0; t < NR; t++)
-            xout[(size_t)r * NR + t] = copy ? xin[(size_t)r * NR + t] : y[(size_t)r * NR + t] - acc[t];
+        for (int t = 0; t < NR; t++) xout[(size_t)r * NR + t] = base[t] - acc[t];
 }
 
 // ---------------------------------------------------------------- coupling
@@ -965,19 +963,25 @@ hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int
 #undef SPIKE_RED
 }
 
+template <int G, int U, int NR>
+static void jacobi_go(int nrows, const int4 *r4, const int *ci, const double *v, const double *y, const double *xin,
+                      double *xout, hipStream_t s)
+{
+    const long threads = (long)nrows * G;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_jacobi_sweep<G, U, NR>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout);
+}
+
 hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const double *v, const double *y,
                                const double *xin, double *xout, int nr_slots, hipStream_t s)
 {
     if (nrows <= 0) return hipSuccess;
-    constexpr int G = 8, U = 4;
-    const long threads = (long)nrows * G;
-    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
     const int4 *r4 = (const int4 *)rows4;
     switch (nr_slots) {
-    case 1: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 1>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
-    case 2: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 2>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
-    case 4: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 4>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
-    case 8: hipLaunchKernelGGL((k_jacobi_sweep<G, U, 8>), grid, block, 0, s, nrows, r4, ci, v, y, xin, xout); break;
+    case 1: jacobi_go<8, 4, 1>(nrows, r4, ci, v, y, xin, xout, s); break;
+    case 2: jacobi_go<8, 4, 2>(nrows, r4, ci, v, y, xin, xout, s); break;
+    case 4: jacobi_go<8, 4, 4>(nrows, r4, ci, v, y, xin, xout, s); break;
+    case 8: jacobi_go<16, 2, 8>(nrows, r4, ci, v, y, xin, xout, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -137,7 +137,7 @@ void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
     if (!fwd) {
         // values pre-scaled by the row's 1/diag (the right-hand side is
         // scaled on load), so every chunk of a row computes x -= sum val * x
-        row.assign((size_t)nsteps * R, (uint16_t)0xFFFF);
+        row.assign((size_t)nsteps * R, (uint16_t)(k + 1));
         mult.clear();
         for (int s = 0; s < nsteps; s++) {
             for (int slot = 0; slot < (int)steps[s].size(); slot++) {

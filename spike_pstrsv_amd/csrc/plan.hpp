@@ -47,7 +47,7 @@ struct StepStream {
     std::vector<int64_t> part_begin;    // P+1 offsets into the step arrays
     std::vector<double> val;            // nsteps * M * 64  [step][round][lane]
     std::vector<uint16_t> col;          // nsteps * M * 64  local column, pad = k_p
-    std::vector<uint16_t> row;          // nsteps * R       local row, 0xFFFF = empty
+    std::vector<uint16_t> row;          // nsteps * R       local row, k+1 = empty
     std::vector<double> mult;           // unused (values are pre-scaled)
     std::vector<double> late;           // nsteps * R * R   fwd only
 };
