@@ -94,8 +94,9 @@ typedef struct {
                               /* row); < 0 = default (1e-15), 0 = keep all     */
                               /* nonzeros (as the reference)                   */
     int precision;            /* SPIKE_PREC_*                                  */
-    int lanes_per_row;        /* block-solve lanes per row (1..64, power of 2);*/
-                              /* 0 = automatic                                 */
+    int lanes_per_row;        /* block-solve lanes per row slot (1, 2, 4, 8   */
+                              /* or 16; longer rows run as chunks of that many */
+                              /* entries in successive steps); 0 = automatic   */
     int verbose;              /* print the partition log after preprocessing   */
     int reduced_solver;       /* SPIKE_REDUCED_*                               */
     int refine_steps;         /* SPIKE_PREC_FP32_REFINED: refinement steps per */
@@ -104,10 +105,13 @@ typedef struct {
                               /* relative error is <= refine_tol               */
     double refine_tol;        /* target of the calibration; <= 0 = 1e-13       */
     int method;               /* SPIKE_METHOD_*                                */
-    int use_graph;            /* 1: a device-pointer solve whose (b, x, ldb,   */
-                              /* ldx, nrhs) repeats the previous call's is     */
-                              /* captured into a hipGraph once and replayed    */
-                              /* (up to 16 cached per factor)                  */
+    int use_graph;            /* 1: a device-pointer solve of many kernel      */
+                              /* launches (>= 24: level schedules, refinement) */
+                              /* whose (b, x, ldb, ldx, nrhs) repeats the      */
+                              /* previous call's is captured into a hipGraph   */
+                              /* once and replayed (up to 16 cached per        */
+                              /* factor); shorter solves launch directly,      */
+                              /* which costs the device less per solve         */
     int autotune;             /* 1: time the RHS tile candidates at creation   */
 } spike_options_t;
 

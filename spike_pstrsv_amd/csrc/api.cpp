@@ -1280,8 +1280,8 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
         if (ev) HIP_TRY(hipEventRecord(ev[2], s));
         ba.xr = xr;
         if (!ba.cpl_fused)
-            HIP_TRY(spike::launch_coupling(pl.P, f->d_parts, f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
-                                           f->d_cpl_val, xr, ba, f->d_bc, s));
+            HIP_TRY(spike::launch_coupling(pl.P, f->d_parts, f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col, xr, ba,
+                                           f->d_bc, s));
         if (ev) HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
     } else if (ev) {
@@ -1423,6 +1423,21 @@ int solve_all(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, 
 
 // Device-pointer solve through the graph cache.  Returns 1 when no graph is
 // used for this call (the caller launches eagerly).
+// Kernel launches of one solve (estimate).  A replayed graph costs the
+// device several microseconds more than the same kernels launched one by one
+// (measured on gfx950: about 6 us per solve on a 1M-row Spike solve of ten
+// launches), so the graph cache only serves solves of many launches, where
+// host launch overhead would exceed the device time.
+long solve_launches(const spike_factor_s *f, int nrhs)
+{
+    const int per_call = std::max(f->max_nr, 1);
+    const long chunks = (nrhs + per_call - 1) / per_call;
+    long per = f->method == SPIKE_METHOD_LEVELS ? (long)f->nlev_full + 2 : 3L + std::max(f->sweeps, 1);
+    if (f->fp32) per = per * (1 + f->refine_steps) + 2L * f->refine_steps;
+    return chunks * per;
+}
+constexpr long kGraphMinLaunches = 24;
+
 int solve_graph(spike_factor_s *f, int nrhs, const double *b, int ldb, double *x, int ldx, hipStream_t s)
 {
     const spike_factor_s::GraphKey key{b, x, ldb, ldx, nrhs};
@@ -1482,7 +1497,7 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
         int rc = ensure_level_ws(f, nrhs);
         if (rc) return rc;
     }
-    if (bdev && xdev && !timed && f->opts.use_graph) {
+    if (bdev && xdev && !timed && f->opts.use_graph && solve_launches(f, nrhs) >= kGraphMinLaunches) {
         int rc = solve_graph(f, nrhs, b, ldb, x, ldx, s);
         if (rc != 1) return rc;
     }
@@ -1782,7 +1797,7 @@ std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vec
         if (G <= 0) {
             std::vector<int> len(M.n);
             for (int r = 0; r < M.n; r++) len[r] = M.ia[r + 1] - M.ia[r] - 1;
-            G = spike::choose_lanes(len);
+            G = spike::choose_block_lanes(len);
         }
         key[j] = {slots, G};
     }

@@ -307,25 +307,21 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 t[0] = group_sum<G>(t[0]);
                 xs[li == 0 ? row : (unsigned)(k + 2 + lane)] = t[0];
             } else {
-                T bv[NR];
+                // Several RHS: the same arithmetic per RHS as one RHS above
+                // (so a column gives the same bits in any tile); every lane
+                // of a row slot holds the same sums and writes them to the
+                // row (slot k+1 for an empty slot): no branch on the chain.
+                const unsigned arow = li == 0 ? row : (unsigned)k;
 #pragma unroll
-                for (int r = 0; r < NR; r++) bv[r] = xs[row * NR + r];
-                // keep the accumulator reads next to the operand reads (their
-                // latency then overlaps instead of following the reduction)
-#pragma unroll
-                for (int r = 0; r < NR; r++) asm volatile("" ::"v"(bv[r]));
+                for (int r = 0; r < NR; r++) t[r] = xs[arow * NR + r];
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-                    t[r] = (T)0;
 #pragma unroll
                     for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
                     t[r] = group_sum<G>(t[r]);
                 }
-                // Every lane of a row slot holds the same sum and accumulator,
-                // so all of them write the same value to the row (slot k+1
-                // for an empty slot): no branch on the step chain.
 #pragma unroll
-                for (int r = 0; r < NR; r++) xs[row * NR + r] = bv[r] + t[r];
+                for (int r = 0; r < NR; r++) xs[row * NR + r] = t[r];
             }
         };
         // Refill slot d with step sidx + D (clamped to the last step), after
@@ -649,27 +645,29 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
 // ---------------------------------------------------------------- coupling
 
 // One workgroup per partition: threads over its coupling rows.
-template <int NR>
+template <int NR, class T>
 __global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ parts, const int *__restrict__ rows,
                                                   const int *__restrict__ ptr, const int *__restrict__ col,
-                                                  const double *__restrict__ val, const double *__restrict__ xred,
-                                                  BlockArgs A, double *__restrict__ bc)
+                                                  const double *__restrict__ xred, BlockArgs A,
+                                                  double *__restrict__ bc)
 {
+#pragma clang fp contract(off)
+    const T *__restrict__ val = VT<T>::cpl(A);
     const DevPart pt = parts[blockIdx.x];
     const RhsMap m = rhs_map(A, pt);
     for (int c = pt.cbeg + threadIdx.x; c < pt.cbeg + pt.ccnt; c += blockDim.x) {
         const int g = rows[c];
-        double acc[NR];
+        T acc[NR];
 #pragma unroll
-        for (int t = 0; t < NR; t++) acc[t] = 0.0;
+        for (int t = 0; t < NR; t++) acc[t] = (T)0;
         for (int j = ptr[c]; j < ptr[c + 1]; j++) {
-            const double w = val[j];
+            const T w = val[j];
             const size_t x0 = (size_t)col[j] * NR;
 #pragma unroll
-            for (int t = 0; t < NR; t++) acc[t] = fma(w, xred[x0 + t], acc[t]);
+            for (int t = 0; t < NR; t++) acc[t] += w * (T)xred[x0 + t];
         }
 #pragma unroll
-        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = m.bv(t, g) - acc[t];
+        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = (double)((T)m.bv(t, g) - acc[t]);
     }
 }
 
@@ -976,30 +974,40 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
                                const double *xin, double *xout, int nr_slots, hipStream_t s)
 {
     if (nrows <= 0) return hipSuccess;
+    // one lane layout for every RHS count: a column gets the same bits
+    // whatever else is solved with it
     const int4 *r4 = (const int4 *)rows4;
     switch (nr_slots) {
     case 1: jacobi_go<8, 4, 1>(nrows, r4, ci, v, y, xin, xout, s); break;
     case 2: jacobi_go<8, 4, 2>(nrows, r4, ci, v, y, xin, xout, s); break;
     case 4: jacobi_go<8, 4, 4>(nrows, r4, ci, v, y, xin, xout, s); break;
-    case 8: jacobi_go<16, 2, 8>(nrows, r4, ci, v, y, xin, xout, s); break;
+    case 8: jacobi_go<8, 4, 8>(nrows, r4, ci, v, y, xin, xout, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class T>
+static hipError_t coupling_t(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr,
+                             const int *cpl_col, const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
+{
+    const dim3 grid(P), block(256);
+    switch (a.nr_slots) {
+    case 1: hipLaunchKernelGGL((k_coupling<1, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
+    case 2: hipLaunchKernelGGL((k_coupling<2, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
+    case 4: hipLaunchKernelGGL((k_coupling<4, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
+    case 8: hipLaunchKernelGGL((k_coupling<8, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
+                           const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
 {
     if (P <= 0) return hipSuccess;
-    const dim3 grid(P), block(256);
-    switch (a.nr_slots) {
-    case 1: hipLaunchKernelGGL(k_coupling<1>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
-    case 2: hipLaunchKernelGGL(k_coupling<2>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
-    case 4: hipLaunchKernelGGL(k_coupling<4>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
-    case 8: hipLaunchKernelGGL(k_coupling<8>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return a.fp32 ? coupling_t<float>(P, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc, s)
+                  : coupling_t<double>(P, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc, s);
 }
 
 hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int *ja, const double *a_,

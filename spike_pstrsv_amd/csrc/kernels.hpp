@@ -100,9 +100,12 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
 
 // bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c and
 // right-hand side (bc, xred interleaved with a.nr_slots values per index); b
-// is addressed per partition as in the block solve (a.b, a.ldb, a.nr).
+// is addressed per partition as in the block solve (a.b, a.ldb, a.nr).  The
+// values are a.cpl_val (a.cpl_val32 for FP32 factors); products are rounded
+// and summed in entry order in that precision, exactly as pass 2 does when it
+// fuses the coupling update, so both forms give the same bits.
 hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s);
+                           const double *xred, const BlockArgs &a, double *bc, hipStream_t s);
 
 // Spike build: for every partition with kcnt > 0, W_p = D0^-1 C[:, K_p] on
 // rows [f, g] (row-major, ld = kcnt) using the lower-form CSR (ia, ja, a).

@@ -209,6 +209,18 @@ int choose_lanes(const std::vector<int> &row_len)
     return G;
 }
 
+int choose_block_lanes(const std::vector<int> &row_len)
+{
+    if (row_len.empty()) return 1;
+    std::vector<int> v(row_len);
+    size_t q = (v.size() * 9) / 10;
+    std::nth_element(v.begin(), v.begin() + q, v.end());
+    const int p90 = v[q];
+    int G = 1;
+    while (G * 2 * 3 <= p90 && G < 16) G <<= 1;
+    return G;
+}
+
 std::string build_plan(char uplo, int n, const double *a, const int *ia, const int *ja,
                        const PlanOptions &opt, Plan &pl)
 {
@@ -370,7 +382,7 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
             inblock_len[i] = c;
         }
     }
-    int G = opt.lanes_per_row > 0 ? opt.lanes_per_row : choose_lanes(inblock_len);
+    int G = opt.lanes_per_row > 0 ? opt.lanes_per_row : choose_block_lanes(inblock_len);
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) return "lanes_per_row must be 1, 2, 4, 8 or 16";
     const int M = 1;
     const bool fwd = opt.forward && G == 16;

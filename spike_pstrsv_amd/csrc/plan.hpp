@@ -103,8 +103,16 @@ struct PlanOptions {
 std::string build_plan(char uplo, int n, const double *a, const int *ia, const int *ja,
                        const PlanOptions &opt, Plan &plan);
 
-// Chooses lanes-per-row G for a pass from the row-length statistics.
+// Lanes per row of the level kernel: the smallest power of two (<= 16) that
+// covers the 90th percentile of the row lengths.
 int choose_lanes(const std::vector<int> &row_len);
+// Lanes per row slot of the block solve, whose rows are split into chunks of
+// G entries scheduled as separate steps: the largest power of two (<= 16)
+// with 2G <= p90 / 3, i.e. rows of about three to five chunks.  Measured on
+// gfx950 (one RHS): 16 entries per row runs fastest at G = 4 (153 us per
+// solve on a 1M-row banded matrix, against 157 at G = 8 and 165 at G = 16);
+// two- and three-entry rows (2D / 3D Laplacians) at G = 1.
+int choose_block_lanes(const std::vector<int> &row_len);
 
 // Level (longest dependency chain, 0-based) of each row of a lower triangular
 // CSR given as (rp, ci) over rows [0, m); entries must point to earlier rows.

@@ -226,7 +226,7 @@ METHODS = {"auto": 0, "spike": 1, "levels": 2}
 
 def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False,
             reduced_solver: str = "auto", precision: str = "fp64", refine_steps: int = -1,
-            refine_tol: float = 0.0, method: str = "auto") -> Options:
+            refine_tol: float = 0.0, method: str = "auto", use_graph: bool = True) -> Options:
     o = Options()
     lib.spike_options_default(C.byref(o))
     o.partitions = partitions
@@ -238,6 +238,7 @@ def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0,
     o.refine_steps = refine_steps
     o.refine_tol = refine_tol
     o.method = METHODS[method]
+    o.use_graph = int(use_graph)
     return o
 
 
@@ -247,12 +248,12 @@ class Factor:
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
                  lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto",
                  precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0,
-                 method: str = "auto", starts=None):
+                 method: str = "auto", starts=None, use_graph: bool = True):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()
         o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver, precision, refine_steps,
-                    refine_tol, method)
+                    refine_tol, method, use_graph)
         if starts is None:
             _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
                                            ja.ctypes.data, C.byref(o)))
