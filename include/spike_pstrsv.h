@@ -111,7 +111,10 @@ typedef struct {
                               /* previous call's is captured into a hipGraph   */
                               /* once and replayed (up to 16 cached per        */
                               /* factor); shorter solves launch directly,      */
-                              /* which costs the device less per solve         */
+                              /* which costs the device less per solve;        */
+                              /* 2: every such repeated solve is captured      */
+                              /* (least host time per solve; the serving loop  */
+                              /* uses it); 0: never                            */
     int autotune;             /* 1: time the RHS tile candidates at creation   */
 } spike_options_t;
 

@@ -1497,7 +1497,8 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
         int rc = ensure_level_ws(f, nrhs);
         if (rc) return rc;
     }
-    if (bdev && xdev && !timed && f->opts.use_graph && solve_launches(f, nrhs) >= kGraphMinLaunches) {
+    if (bdev && xdev && !timed && f->opts.use_graph > 0 &&
+        (f->opts.use_graph >= 2 || solve_launches(f, nrhs) >= kGraphMinLaunches)) {
         int rc = solve_graph(f, nrhs, b, ldb, x, ldx, s);
         if (rc != 1) return rc;
     }

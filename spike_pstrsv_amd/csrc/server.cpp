@@ -159,19 +159,33 @@ int spike_server_s::launch(const Job &j)
     return hipEventRecord(j.ev, s) == hipSuccess ? SPIKE_OK : SPIKE_ERR_DEVICE;
 }
 
-// Dispatches the requests of the oldest window (mu held on entry and exit;
-// released while launching).  Solves beyond the free slots stay pending.
+// Dispatches every request whose batching window has closed: the oldest
+// window's requests and, when solves are backed up, every older-than-a-window
+// request behind them, so a backlog is merged into multi-RHS solves (mu held
+// on entry and exit; released while launching).  Matrices are served in the
+// order of their oldest request; solves beyond the free slots stay pending.
 void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
 {
     windows++;
-    const clk::time_point w_end = pending.front().t_submit + std::chrono::nanoseconds((int64_t)(opts.window_us * 1e3));
-    std::map<int, std::vector<Request>> by_mat;
+    const auto win = std::chrono::nanoseconds((int64_t)(opts.window_us * 1e3));
+    const clk::time_point cutoff = std::max(pending.front().t_submit + win, clk::now() - win);
+    std::vector<std::pair<int, std::vector<Request>>> by_mat;
+    std::map<int, size_t> slot_of;
     std::deque<Request> rest;
     while (!pending.empty()) {
         Request r = pending.front();
         pending.pop_front();
-        if (r.t_submit <= w_end) by_mat[r.id].push_back(r);
-        else rest.push_back(r);
+        if (r.t_submit > cutoff) {
+            rest.push_back(r);
+            continue;
+        }
+        auto it = slot_of.find(r.id);
+        if (it == slot_of.end()) {
+            slot_of[r.id] = by_mat.size();
+            by_mat.emplace_back(r.id, std::vector<Request>{r});
+        } else {
+            by_mat[it->second].second.push_back(r);
+        }
     }
     std::vector<Job> jobs;
     std::vector<Request> keep;
@@ -325,8 +339,15 @@ int spike_server_add(spike_server_t s, char uplo, int n, const double *a, const 
 {
     if (!s || !id) return sfail(SPIKE_ERR_ARG, "NULL argument");
     *id = -1;
+    // a served factor replays captured graphs for every solve (one launch
+    // per solve keeps the dispatcher thread's host time small) unless the
+    // caller turned graphs off
+    spike_options_t fopt;
+    if (fo) fopt = *fo;
+    else spike_options_default(&fopt);
+    if (fopt.use_graph == 1) fopt.use_graph = 2;
     spike_factor_t f = nullptr;
-    int rc = spike_factor_create(&f, uplo, n, a, ia, ja, fo);
+    int rc = spike_factor_create(&f, uplo, n, a, ia, ja, &fopt);
     if (rc != SPIKE_OK) return sfail(rc, spike_last_error());
     spike_stats_t st;
     spike_factor_stats(f, &st);

@@ -226,7 +226,7 @@ METHODS = {"auto": 0, "spike": 1, "levels": 2}
 
 def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False,
             reduced_solver: str = "auto", precision: str = "fp64", refine_steps: int = -1,
-            refine_tol: float = 0.0, method: str = "auto", use_graph: bool = True) -> Options:
+            refine_tol: float = 0.0, method: str = "auto", use_graph: int = 1) -> Options:
     o = Options()
     lib.spike_options_default(C.byref(o))
     o.partitions = partitions
@@ -248,7 +248,7 @@ class Factor:
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
                  lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto",
                  precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0,
-                 method: str = "auto", starts=None, use_graph: bool = True):
+                 method: str = "auto", starts=None, use_graph: int = 1):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()

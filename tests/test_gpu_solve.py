@@ -255,20 +255,20 @@ def test_factor_save_load(native, tmp_path, case):
     assert relerr(X1[0], orc.trsv(uplo, m.a, m.ia, m.ja, B[0])) <= TOL
 
 
-@pytest.mark.parametrize("method", ["levels", "spike"])
-def test_graph_replays_match_direct_launches(native, method):
+@pytest.mark.parametrize("method,use_graph", [("levels", 1), ("spike", 1), ("spike", 2)])
+def test_graph_replays_match_direct_launches(native, method, use_graph):
     """Repeated device-pointer solves: the level schedule (many launches) is
     captured into a hipGraph on the second call and replayed after that; the
-    Spike solve (ten launches) keeps launching directly.  Every call, with
-    eager device work and synchronisations between them, must give the bits
-    of a solve made without the graph cache."""
+    Spike solve (ten launches) keeps launching directly unless use_graph = 2.
+    Every call, with eager device work and synchronisations between them,
+    must give the bits of a solve made without the graph cache."""
     import torch
 
     m = gen.lap3d(32)
     B = gen.rhs(m.n, 3, seed=gen.SEED_BASE + 260)
     f0 = native.Factor("L", m.a, m.ia, m.ja, partitions=16, method=method, use_graph=False)
     want = f0.solve(B, nrhs=3)
-    f = native.Factor("L", m.a, m.ia, m.ja, partitions=16, method=method)
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=16, method=method, use_graph=use_graph)
     db = torch.from_numpy(B).cuda()
     dx = torch.zeros_like(db)
     s = torch.cuda.current_stream().cuda_stream
