@@ -113,7 +113,6 @@ struct spike_factor_s {
     // SPIKE_PREC_FP32_REFINED: FP32 step and coupling values, the FP64
     // lower-form CSR for the residual, a residual workspace (n x max_nr)
     bool fp32 = false;
-    float *d_cpl_val32 = nullptr;
     const double *d_a = nullptr;
     const int *d_ia = nullptr, *d_ja = nullptr;
     double *d_r = nullptr;
@@ -284,7 +283,7 @@ bool fuse_ok(const spike_factor_s *f, int slots, int t)
 {
     const size_t lim = spike::block_solve_lds_limit();
     const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
-    const size_t x_per = (size_t)(f->plan.kmax + spike::XS_PAD) * vb, c_per = (size_t)f->cmax * vb;
+    const size_t x_per = (size_t)(f->plan.kmax + spike::XS_PAD) * vb + 8, c_per = (size_t)f->cmax * sizeof(double);
     if (!f->cpl_fused || (x_per + c_per) * t > lim) return false;
     return tile_rounds(f, slots, t, (x_per + c_per) * t) == tile_rounds(f, slots, t, x_per * t);
 }
@@ -313,8 +312,8 @@ void choose_rhs_slots(spike_factor_s *f)
     f->lds_cu = lds_cu;
     const size_t vb = f->fp32 ? sizeof(float) : sizeof(double);
     const size_t x_per = (size_t)(pl.kmax + spike::XS_PAD) * vb;
-    const size_t c_per = (size_t)f->cmax * vb;
-    f->cpl_fused = x_per + c_per <= lim;
+    const size_t c_per = (size_t)f->cmax * sizeof(double);
+    f->cpl_fused = x_per + 8 + c_per <= lim;
     static const double step_cost[4] = {1.0, 1.22, 2.1, 4.2};
     for (int li = 0; li < 4; li++) {
         f->tile_at[li] = 0;
@@ -516,6 +515,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     if (const char *e = std::getenv("SPIKE_PSTRSV_FORWARD")) po.forward = std::atoi(e) != 0 && !f->fp32;
     po.P = P;
     po.lanes_per_row = f->opts.lanes_per_row;
+    po.fp32 = f->fp32;
     if (const char *e = std::getenv("SPIKE_PSTRSV_RING")) f->ring = std::atoi(e) >= 16 ? 16 : 8;
     std::string err = spike::build_plan(uplo, n, a, ia, ja, po, f->plan);
     if (!err.empty()) return fail(SPIKE_ERR_MATRIX, err);
@@ -580,7 +580,6 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         return rc;
     if ((rc = upload_steps(f, pl.steps, f->steps, f->fp32)) || (rc = f->upload(&f->d_inv_diag, pl.inv_diag)))
         return rc;
-    if (f->fp32 && (rc = f->upload(&f->d_cpl_val32, to_f32(pl.cpl_val)))) return rc;
 
     const int nred = (int)pl.dep_rows.size();
     for (int t = 0; t < P; t++) {
@@ -887,7 +886,7 @@ std::vector<std::pair<const char *, void **>> device_fields(spike_factor_s *f)
     return {{"part", (void **)&f->d_parts},          {"deps", (void **)&f->d_dep_rows},
             {"crow", (void **)&f->d_cpl_rows},       {"cptr", (void **)&f->d_cpl_ptr},
             {"ccol", (void **)&f->d_cpl_col},        {"cval", (void **)&f->d_cpl_val},
-            {"cv32", (void **)&f->d_cpl_val32},      {"sval", (void **)&f->steps.val},
+            {"sval", (void **)&f->steps.val},
             {"scol", (void **)&f->steps.col},        {"srow", (void **)&f->steps.row},
             {"smul", (void **)&f->steps.mult},       {"slat", (void **)&f->steps.late},
             {"spb_", (void **)&f->steps.part_begin}, {"sv32", (void **)&f->steps.val32},
@@ -1152,7 +1151,6 @@ spike::BlockArgs make_args(spike_factor_s *f, const double *b, int ldb, double *
     ba.ldb = ldb;
     ba.ldx = ldx;
     ba.fp32 = f->fp32 ? 1 : 0;
-    ba.cpl_val32 = f->d_cpl_val32;
     ba.accum = accum;
     return ba;
 }
@@ -1280,8 +1278,8 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
         if (ev) HIP_TRY(hipEventRecord(ev[2], s));
         ba.xr = xr;
         if (!ba.cpl_fused)
-            HIP_TRY(spike::launch_coupling(pl.P, f->d_parts, f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col, xr, ba,
-                                           f->d_bc, s));
+            HIP_TRY(spike::launch_coupling(pl.P, f->d_parts, f->d_cpl_rows, f->d_cpl_ptr, f->d_cpl_col,
+                                           f->d_cpl_val, xr, ba, f->d_bc, s));
         if (ev) HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(spike::launch_block_solve(2, pl.P, ba, s));
     } else if (ev) {
@@ -1786,7 +1784,7 @@ int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpe
 // Groups: right-hand-side slots (next power of two >= nrhs) x lanes per row
 // (from the p90 off-diagonal row length, or the option); a group is split
 // where it would pass 1.5e9 entries.
-std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vector<int> &idx, int lanes)
+std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vector<int> &idx, int lanes, bool fp32)
 {
     std::vector<GroupSpec> out;
     std::vector<std::pair<int, int>> key(idx.size());
@@ -1798,7 +1796,7 @@ std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vec
         if (G <= 0) {
             std::vector<int> len(M.n);
             for (int r = 0; r < M.n; r++) len[r] = M.ia[r + 1] - M.ia[r] - 1;
-            G = spike::choose_block_lanes(len);
+            G = fp32 ? spike::choose_lanes(len) : spike::choose_block_lanes(len);
         }
         key[j] = {slots, G};
     }
@@ -1839,7 +1837,7 @@ int batch_run(spike_batch_s *bt, const double *b, double *x, hipStream_t s)
 int batch_build(spike_batch_s *bt, const spike_batch_matrix_t *m, const std::vector<int> &idx, const int *rpp,
                 int refine_steps)
 {
-    std::vector<GroupSpec> specs = plan_groups(m, idx, bt->opts.lanes_per_row);
+    std::vector<GroupSpec> specs = plan_groups(m, idx, bt->opts.lanes_per_row, bt->opts.precision == SPIKE_PREC_FP32_REFINED);
     for (const GroupSpec &g : specs) {
         spike_factor_s *f = nullptr;
         int rc = build_group(bt, m, g, rpp[__builtin_ctz((unsigned)g.slots)], bt->opts.precision, refine_steps,
@@ -2248,6 +2246,7 @@ int spike_analyze(char uplo, int n, const double *a, const int *ia, const int *j
     int P = o.partitions > 0 ? o.partitions : default_partitions(uplo, n, ia, ja, kmax_limit_for_device());
     po.P = std::min(P, n);
     po.lanes_per_row = o.lanes_per_row;
+    po.fp32 = o.precision == SPIKE_PREC_FP32_REFINED;
     Plan pl;
     std::string err = spike::build_plan(uplo, n, a, ia, ja, po, pl);
     if (!err.empty()) return fail(SPIKE_ERR_MATRIX, err);

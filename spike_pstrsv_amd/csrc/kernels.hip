@@ -112,7 +112,6 @@ __device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, unsigned voff
 template <class T> struct VT;
 template <> struct VT<double> {
     static __device__ __forceinline__ const double *val(const BlockArgs &A) { return A.st.val; }
-    static __device__ __forceinline__ const double *cpl(const BlockArgs &A) { return A.cpl_val; }
     static __device__ __forceinline__ double load(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s)
     {
         return buf_f64(r, v, s);
@@ -120,7 +119,6 @@ template <> struct VT<double> {
 };
 template <> struct VT<float> {
     static __device__ __forceinline__ const float *val(const BlockArgs &A) { return A.st.val32; }
-    static __device__ __forceinline__ const float *cpl(const BlockArgs &A) { return A.cpl_val32; }
     static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, unsigned v, unsigned s)
     {
         return buf_f32(r, v, s);
@@ -163,14 +161,24 @@ __device__ __forceinline__ RhsMap rhs_map(const BlockArgs &A, const DevPart &pt)
     return m;
 }
 
+// Bytes of a block-solve x segment (kmax + XS_PAD slots of NR values), rounded
+// up to 8 so that the FP64 coupling scratch can follow it.
+template <class T>
+__host__ __device__ __forceinline__ size_t xs_bytes(int kmax, int NR)
+{
+    return ((size_t)(kmax + XS_PAD) * NR * sizeof(T) + 7) & ~(size_t)7;
+}
+
 // Pass-2 right-hand side of one partition in LDS, NR right-hand sides
 // interleaved (xs[i*NR + r]): b for every row, then b - sum_j c_ij xr[col_j]
-// on the coupling rows (times 1/diag when scaled).  The coupling products are
-// gathered lane-parallel into the LDS scratch prod[] and each coupling row
-// then sums its own products in entry order, so the result does not depend
-// on the lane count.  Right-hand sides r >= nr reuse column nr-1.
+// on the coupling rows (times 1/diag when scaled).  The coupling update is
+// FP64 for FP32 factors too: the products are gathered lane-parallel into the
+// FP64 LDS scratch prod[] and each coupling row then sums its own products in
+// entry order, exactly as k_coupling does, so the result depends neither on
+// the lane count nor on which of the two computes it.  Right-hand sides
+// r >= nr reuse column nr-1.
 template <int NR, class T>
-__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, T *prod, bool scaled,
+__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, double *prod, bool scaled,
                                           int roff, int RS)
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k;
@@ -183,27 +191,27 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
     }
     if (pt.ccnt == 0) return;
     if (A.cpl_fused) {
-        const T *cv = VT<T>::cpl(A);
+        const double *cv = A.cpl_val;
         const int e0 = A.cpl_ptr[pt.cbeg], e1 = A.cpl_ptr[pt.cbeg + pt.ccnt];
 #pragma unroll 4
         for (int e = e0 + lane; e < e1; e += 64) {
-            const T w = cv[e];
+            const double w = cv[e];
             const double *xc = A.xr + (size_t)A.cpl_col[e] * RS + roff;
 #pragma unroll
-            for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * (T)xc[r];
+            for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * xc[r];
         }
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
             const int g = A.cpl_rows[pt.cbeg + c];
             const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
-            T acc[NR];
+            double acc[NR];
 #pragma unroll
-            for (int r = 0; r < NR; r++) acc[r] = (T)0;
+            for (int r = 0; r < NR; r++) acc[r] = 0.0;
             for (int e = A.cpl_ptr[pt.cbeg + c]; e < A.cpl_ptr[pt.cbeg + c + 1]; e++)
 #pragma unroll
                 for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
 #pragma unroll
-            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = ((T)m.bv(roff + r, g) - acc[r]) * sc;
+            for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = (T)(m.bv(roff + r, g) - acc[r]) * sc;
         }
     } else {
         __syncthreads();
@@ -248,7 +256,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, s + i) * sc;
         }
     } else {
-        pass2_rhs<NR, T>(A, pt, xs, xs + (size_t)(A.kmax + XS_PAD) * NR, true, roff, RS);
+        pass2_rhs<NR, T>(A, pt, xs, reinterpret_cast<double *>(smem_blk + xs_bytes<T>(A.kmax, NR)), true, roff, RS);
     }
     if (lane == 0)
 #pragma unroll
@@ -479,7 +487,7 @@ template <int G, int D, int PASS, int NR, class T>
 hipError_t launch_gdt(int P, const BlockArgs &a, hipStream_t s)
 {
     constexpr int M = 1;
-    size_t lds = (size_t)(a.kmax + XS_PAD + (PASS == 2 && a.cpl_fused ? a.cmax : 0)) * NR * sizeof(T);
+    size_t lds = xs_bytes<T>(a.kmax, NR) + (PASS == 2 && a.cpl_fused ? (size_t)a.cmax * NR * sizeof(double) : 0);
     auto kern = k_block_solve<G, M, D, PASS, NR, T>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -645,29 +653,28 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
 // ---------------------------------------------------------------- coupling
 
 // One workgroup per partition: threads over its coupling rows.
-template <int NR, class T>
+template <int NR>
 __global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ parts, const int *__restrict__ rows,
                                                   const int *__restrict__ ptr, const int *__restrict__ col,
-                                                  const double *__restrict__ xred, BlockArgs A,
-                                                  double *__restrict__ bc)
+                                                  const double *__restrict__ val, const double *__restrict__ xred,
+                                                  BlockArgs A, double *__restrict__ bc)
 {
 #pragma clang fp contract(off)
-    const T *__restrict__ val = VT<T>::cpl(A);
     const DevPart pt = parts[blockIdx.x];
     const RhsMap m = rhs_map(A, pt);
     for (int c = pt.cbeg + threadIdx.x; c < pt.cbeg + pt.ccnt; c += blockDim.x) {
         const int g = rows[c];
-        T acc[NR];
+        double acc[NR];
 #pragma unroll
-        for (int t = 0; t < NR; t++) acc[t] = (T)0;
+        for (int t = 0; t < NR; t++) acc[t] = 0.0;
         for (int j = ptr[c]; j < ptr[c + 1]; j++) {
-            const T w = val[j];
+            const double w = val[j];
             const size_t x0 = (size_t)col[j] * NR;
 #pragma unroll
-            for (int t = 0; t < NR; t++) acc[t] += w * (T)xred[x0 + t];
+            for (int t = 0; t < NR; t++) acc[t] += w * xred[x0 + t];
         }
 #pragma unroll
-        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = (double)((T)m.bv(t, g) - acc[t]);
+        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = m.bv(t, g) - acc[t];
     }
 }
 
@@ -987,27 +994,19 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
     return hipGetLastError();
 }
 
-template <class T>
-static hipError_t coupling_t(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr,
-                             const int *cpl_col, const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
+hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
+                           const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
 {
+    if (P <= 0) return hipSuccess;
     const dim3 grid(P), block(256);
     switch (a.nr_slots) {
-    case 1: hipLaunchKernelGGL((k_coupling<1, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
-    case 2: hipLaunchKernelGGL((k_coupling<2, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
-    case 4: hipLaunchKernelGGL((k_coupling<4, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
-    case 8: hipLaunchKernelGGL((k_coupling<8, T>), grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc); break;
+    case 1: hipLaunchKernelGGL(k_coupling<1>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
+    case 2: hipLaunchKernelGGL(k_coupling<2>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
+    case 4: hipLaunchKernelGGL(k_coupling<4>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
+    case 8: hipLaunchKernelGGL(k_coupling<8>, grid, block, 0, s, parts, cpl_rows, cpl_ptr, cpl_col, cpl_val, xred, a, bc); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
-}
-
-hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *xred, const BlockArgs &a, double *bc, hipStream_t s)
-{
-    if (P <= 0) return hipSuccess;
-    return a.fp32 ? coupling_t<float>(P, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc, s)
-                  : coupling_t<double>(P, parts, cpl_rows, cpl_ptr, cpl_col, xred, a, bc, s);
 }
 
 hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int *ja, const double *a_,

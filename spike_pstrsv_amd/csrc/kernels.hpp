@@ -56,8 +56,9 @@ struct BlockArgs {
     int kmax;               // largest partition (LDS x segment = kmax+1 doubles)
     int ring;               // step records kept in flight per wavefront (8 or 16)
     const double *inv_diag; // fwd: 1/diag per row (lower form)
-    // pass 2 computes b - C x_K itself when cpl_fused (coupling entries of a
-    // partition fit the LDS scratch of cmax doubles); otherwise bc is used
+    // pass 2 computes b - C x_K itself when cpl_fused (the FP64 coupling
+    // products of a partition fit an LDS scratch of cmax doubles per RHS after
+    // the x segment); otherwise bc is used
     int cpl_fused;
     int cmax;
     const int *cpl_ptr;
@@ -69,10 +70,9 @@ struct BlockArgs {
     // nr_slots interleaved values per index
     int nr, nr_slots;
     int ldb, ldx;
-    // FP32 factors: step values, multipliers and coupling values are FP32 and
-    // the block solve accumulates in FP32 (b, xred, xr and x stay FP64)
+    // FP32 factors: step values and multipliers are FP32 and the block solve
+    // accumulates in FP32 (b, xred, xr, x and the coupling update stay FP64)
     int fp32;
-    const float *cpl_val32;
     // 1: the solve's result is added to x (refinement update x += d)
     int accum;
     // right-hand sides per block-solve wave (divides nr_slots): a launch runs
@@ -101,11 +101,11 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
 // bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c and
 // right-hand side (bc, xred interleaved with a.nr_slots values per index); b
 // is addressed per partition as in the block solve (a.b, a.ldb, a.nr).  The
-// values are a.cpl_val (a.cpl_val32 for FP32 factors); products are rounded
-// and summed in entry order in that precision, exactly as pass 2 does when it
-// fuses the coupling update, so both forms give the same bits.
+// update is FP64 for FP32 factors too; products are rounded and summed in
+// entry order, exactly as pass 2 does when it fuses the coupling update, so
+// both forms give the same bits.
 hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
-                           const double *xred, const BlockArgs &a, double *bc, hipStream_t s);
+                           const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s);
 
 // Spike build: for every partition with kcnt > 0, W_p = D0^-1 C[:, K_p] on
 // rows [f, g] (row-major, ld = kcnt) using the lower-form CSR (ia, ja, a).

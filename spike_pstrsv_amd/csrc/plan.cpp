@@ -382,7 +382,8 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
             inblock_len[i] = c;
         }
     }
-    int G = opt.lanes_per_row > 0 ? opt.lanes_per_row : choose_block_lanes(inblock_len);
+    int G = opt.lanes_per_row > 0 ? opt.lanes_per_row
+                                  : (opt.fp32 ? choose_lanes(inblock_len) : choose_block_lanes(inblock_len));
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) return "lanes_per_row must be 1, 2, 4, 8 or 16";
     const int M = 1;
     const bool fwd = opt.forward && G == 16;
