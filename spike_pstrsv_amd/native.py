@@ -317,11 +317,11 @@ class Factor:
             pass
 
 
-def analyze(uplo: str, a, ia, ja, partitions: int = 0, lanes_per_row: int = 0) -> dict:
+def analyze(uplo: str, a, ia, ja, partitions: int = 0, lanes_per_row: int = 0, precision: str = "fp64") -> dict:
     """Host-only symbolic analysis (no device needed)."""
     a, ia, ja = _csr(a, ia, ja)
     st = Stats()
-    o = options(partitions, -1.0, lanes_per_row)
+    o = options(partitions, -1.0, lanes_per_row, precision=precision)
     _check(lib.spike_analyze(uplo.encode()[:1], len(ia) - 1, a.ctypes.data, ia.ctypes.data, ja.ctypes.data,
                              C.byref(o), C.byref(st)))
     return st.as_dict()

@@ -75,6 +75,26 @@ def test_analyze_lap2d_depth():
     assert st["n_reduced"] == 7 * 256
 
 
+@pytest.mark.parametrize("case,precision,lanes", [
+    ("banded", "fp64", 4), ("lap2d", "fp64", 1), ("lap3d", "fp64", 1),
+    ("banded", "fp32_refined", 16), ("lap2d", "fp32_refined", 2)])
+def test_block_lanes_choice(case, precision, lanes):
+    """Automatic lanes per row slot: FP64 factors split rows into chunks of
+    about p90/3..p90/6 entries (16-entry banded rows -> 4 lanes, the 2- and
+    3-entry Laplacian rows -> 1); FP32 factors cover whole rows (the smallest
+    power of two >= p90), so each row's FP32 partial sum is rounded once."""
+    from spike_pstrsv_amd import native
+
+    m = {"banded": lambda: gen.banded(1 << 14), "lap2d": lambda: gen.lap2d(64),
+         "lap3d": lambda: gen.lap3d(16)}[case]()
+    st = native.analyze("L", m.a, m.ia, m.ja, partitions=8, precision=precision)
+    assert st["lanes_per_row"] == lanes
+    # a row of len entries spans ceil(len / lanes) chunks, each >= 1 step
+    # after the previous one: the schedule is at least that deep
+    lens = np.diff(m.ia) - 1
+    assert st["steps_pass1"] >= int(np.max((lens + lanes - 1) // lanes))
+
+
 @pytest.mark.parametrize("bad", ["unsorted", "nodiag", "emptyrow", "ia0", "range"])
 def test_invalid_csr_rejected(bad):
     from spike_pstrsv_amd import native
