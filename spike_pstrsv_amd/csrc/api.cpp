@@ -133,14 +133,21 @@ struct spike_factor_s {
     int batch_max_nrhs = 0;
     int64_t vec_len = 0;        // batch: packed length of b and x
     // SPIKE_METHOD_LEVELS: rows grouped by dependency level (lev_ptr on the
-    // host delimits the launches), the lower-form CSR and 1/diag on the
-    // device, a correction workspace for refinement
+    // host delimits the launches), their records and strictly lower entries
+    // in level order and 1/diag on the device (plus the FP64 lower-form CSR
+    // for the residual of FP32 factors), a correction workspace for
+    // refinement
     int method = SPIKE_METHOD_SPIKE;
     bool caller_parts = false;  // partitions given by the caller: Spike only
     int nlev_full = 0;
     std::vector<int> lev_ptr;
-    int *d_lrows = nullptr;
-    float *d_a32 = nullptr, *d_invd32 = nullptr;
+    int *d_lrec = nullptr;      // int4 per level-ordered position: row, entries
+    int *d_lja = nullptr;       // strictly lower entries in level order
+    double *d_lval = nullptr;
+    float *d_lval32 = nullptr, *d_invd32 = nullptr;
+    bool sync_free = true;      // multi-RHS: one sync-free launch instead of one per level
+    int spin_limit = spike::kSpinLimit;
+    int *d_abort = nullptr;     // its abort word
     double *d_dlt = nullptr;
     int lev_G = 1;
     // interleaved right-hand sides / solutions of multi-RHS level solves
@@ -443,26 +450,70 @@ int upload_row_blocks(spike_factor_s *f, const std::vector<DevPart> &parts)
 }
 
 // Device data of the level-scheduled method.
+// Sync-free switch (SPIKE_PSTRSV_SYNC_FREE=0 selects one launch per level)
+// and the abort word it needs (every solve clears it with a kernel).
+int setup_sync_free(spike_factor_s *f)
+{
+    const char *e = std::getenv("SPIKE_PSTRSV_SYNC_FREE");
+    f->sync_free = !(e && std::atoi(e) == 0);
+    // test hook: a small poll limit forces the give-up path
+    if (const char *l = std::getenv("SPIKE_PSTRSV_SPIN_LIMIT")) f->spin_limit = std::max(0, std::atoi(l));
+    if (!f->sync_free || f->d_abort) return SPIKE_OK;
+    int rc;
+    if ((rc = f->dalloc(&f->d_abort, 1))) return rc;
+    HIP_TRY(hipMemset(f->d_abort, 0, sizeof(int)));
+    return SPIKE_OK;
+}
+
 int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
 {
     Plan &pl = f->plan;
     const int n = pl.n;
     int rc;
-    int *d_ia, *d_ja;
-    double *d_a, *d_invd;
-    if ((rc = f->upload(&f->d_lrows, rows)) || (rc = f->upload(&d_ia, pl.ia)) || (rc = f->upload(&d_ja, pl.ja)) ||
-        (rc = f->upload(&d_a, pl.a)) || (rc = f->upload(&d_invd, pl.inv_diag)))
+    // Row records and strictly lower entries in level order.  Every level
+    // starts at a multiple of 64 positions (padding records have row -1), so
+    // no wavefront holds rows of two levels: in the sync-free solve a row may
+    // wait for another wavefront's rows, never for its own wavefront's.
+    std::vector<int> rec, lja;
+    std::vector<double> lval;
+    lja.reserve((size_t)(pl.nnz - n));
+    lval.reserve((size_t)(pl.nnz - n));
+    std::vector<int> padded_ptr(1, 0);
+    for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++) {
+        for (int q = f->lev_ptr[l]; q < f->lev_ptr[l + 1]; q++) {
+            const int row = rows[q];
+            const int eb = (int)lja.size();
+            for (int e = pl.ia[row]; e < pl.ia[row + 1] - 1; e++) {   // the diagonal is last
+                lja.push_back(pl.ja[e]);
+                lval.push_back(pl.a[e]);
+            }
+            rec.insert(rec.end(), {row, eb, (int)lja.size(), 0});
+        }
+        while ((rec.size() / 4) % 64) rec.insert(rec.end(), {-1, 0, 0, 0});
+        padded_ptr.push_back((int)(rec.size() / 4));
+    }
+    f->lev_ptr = padded_ptr;
+    if ((rc = f->upload(&f->d_lrec, rec)) || (rc = f->upload(&f->d_lja, lja)) ||
+        (rc = f->upload(&f->d_inv_diag, pl.inv_diag)))
         return rc;
-    f->d_a = d_a;
-    f->d_ia = d_ia;
-    f->d_ja = d_ja;
-    f->d_inv_diag = d_invd;
-    if (f->fp32 && ((rc = f->upload(&f->d_a32, to_f32(pl.a))) || (rc = f->upload(&f->d_invd32, to_f32(pl.inv_diag)))))
+    if (f->fp32) {
+        // FP32 values for the solves; the lower-form CSR in FP64 for the residual
+        int *d_ia, *d_ja;
+        double *d_a;
+        if ((rc = f->upload(&f->d_lval32, to_f32(lval))) || (rc = f->upload(&f->d_invd32, to_f32(pl.inv_diag))) ||
+            (rc = f->upload(&d_ia, pl.ia)) || (rc = f->upload(&d_ja, pl.ja)) || (rc = f->upload(&d_a, pl.a)))
+            return rc;
+        f->d_a = d_a;
+        f->d_ia = d_ia;
+        f->d_ja = d_ja;
+    } else if ((rc = f->upload(&f->d_lval, lval))) {
         return rc;
+    }
     std::vector<int> len(n);
     for (int i = 0; i < n; i++) len[i] = pl.ia[i + 1] - pl.ia[i] - 1;
     f->lev_G = f->opts.lanes_per_row > 0 ? std::min(f->opts.lanes_per_row, 16) : spike::choose_lanes(len);
     f->max_nr = 32;
+    if ((rc = setup_sync_free(f))) return rc;
     {
         DevPart whole{};
         whole.s = 0;
@@ -867,7 +918,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 // File layout: magic, a block of scalars, then tagged records
 // {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
 // stored by content; workspaces only by size (re-allocated on load).
-constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '2'};
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '3'};
 
 struct FactorScalars {
     spike_options_t opts;
@@ -895,8 +946,9 @@ std::vector<std::pair<const char *, void **>> device_fields(spike_factor_s *f)
             {"rv__", (void **)&f->d_rv},             {"rlst", (void **)&f->d_rlist},
             {"rlev", (void **)&f->d_levptr},         {"swrw", (void **)&f->d_sweep_rows},
             {"a___", (void **)&f->d_a},              {"ia__", (void **)&f->d_ia},
-            {"ja__", (void **)&f->d_ja},             {"lrow", (void **)&f->d_lrows},
-            {"a32_", (void **)&f->d_a32},            {"id32", (void **)&f->d_invd32},
+            {"ja__", (void **)&f->d_ja},             {"lrec", (void **)&f->d_lrec},
+            {"lja_", (void **)&f->d_lja},            {"lval", (void **)&f->d_lval},
+            {"lv32", (void **)&f->d_lval32},         {"id32", (void **)&f->d_invd32},
             {"res_", (void **)&f->d_res}};
 }
 std::vector<std::pair<const char *, double **>> workspace_fields(spike_factor_s *f)
@@ -1104,6 +1156,8 @@ int factor_load(spike_factor_s *f, const char *path)
             }
         if (!done) return fail(SPIKE_ERR_ARG, std::string("unknown record ") + tag);
     }
+    if (f->method == SPIKE_METHOD_LEVELS)
+        if (int rc = setup_sync_free(f)) return rc;
     f->stats.device_bytes = f->device_bytes;
     return SPIKE_OK;
 }
@@ -1189,20 +1243,23 @@ int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx
         while (nrp < nr) nrp <<= 1;
         if ((size_t)n * nrp > f->il_cap) return fail(SPIKE_ERR_STATE, "level workspaces not allocated");
         spike::LevelArgs la{};
-        la.rows = f->d_lrows;
-        la.ia = f->d_ia;
-        la.ja = f->d_ja;
-        la.val = f->d_a;
+        la.rec = f->d_lrec;
+        la.lja = f->d_lja;
+        la.lval = f->d_lval;
         la.invd = f->d_inv_diag;
-        la.val32 = f->d_a32;
+        la.lval32 = f->d_lval32;
         la.invd32 = f->d_invd32;
         la.fp32 = f->fp32 ? 1 : 0;
+        la.spin_limit = f->spin_limit;
         const int flip = f->plan.flip ? 1 : 0;
         if (ev) HIP_TRY(hipEventRecord(ev[0], s));
         HIP_TRY(spike::launch_interleave(n, nr, nrp, flip, b, ldb, f->d_bi, s));
-        for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++)
-            HIP_TRY(spike::launch_level_il(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], nrp, la, f->d_bi,
-                                           f->d_xi, s));
+        if (f->sync_free)
+            HIP_TRY(spike::launch_sync_free(f->lev_ptr.back(), n, nrp, la, f->d_bi, f->d_xi, f->d_abort, s));
+        else
+            for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++)
+                HIP_TRY(spike::launch_level_il(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], nrp, la, f->d_bi,
+                                               f->d_xi, s));
         HIP_TRY(spike::launch_deinterleave(n, nr, nrp, flip, f->d_xi, x, ldx, accum, s));
         if (ev) {
             HIP_TRY(hipEventRecord(ev[1], s));
@@ -1214,12 +1271,11 @@ int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx
         return SPIKE_OK;
     }
     spike::LevelArgs la{};
-    la.rows = f->d_lrows;
-    la.ia = f->d_ia;
-    la.ja = f->d_ja;
-    la.val = f->d_a;
+    la.rec = f->d_lrec;
+    la.lja = f->d_lja;
+    la.lval = f->d_lval;
     la.invd = f->d_inv_diag;
-    la.val32 = f->d_a32;
+    la.lval32 = f->d_lval32;
     la.invd32 = f->d_invd32;
     la.b = b;
     la.ldb = ldb;
@@ -1430,7 +1486,9 @@ long solve_launches(const spike_factor_s *f, int nrhs)
 {
     const int per_call = std::max(f->max_nr, 1);
     const long chunks = (nrhs + per_call - 1) / per_call;
-    long per = f->method == SPIKE_METHOD_LEVELS ? (long)f->nlev_full + 2 : 3L + std::max(f->sweeps, 1);
+    long per = f->method != SPIKE_METHOD_LEVELS ? 3L + std::max(f->sweeps, 1)
+               : (f->sync_free && nrhs >= 2)    ? 4L
+                                                : (long)f->nlev_full + 2;
     if (f->fp32) per = per * (1 + f->refine_steps) + 2L * f->refine_steps;
     return chunks * per;
 }

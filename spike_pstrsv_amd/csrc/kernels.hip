@@ -628,9 +628,14 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
             double w[U], xv[U][NR];
 #pragma unroll
             for (int u = 0; u < U; u++) {
+                // unconditional loads at a clamped index (no branch between
+                // the loads), masked entries then use the row itself
                 const bool ok = j + u * G < e;
-                c[u] = ok ? ci[j + u * G] : r;     // masked: the row's own (valid) entry, not used
-                w[u] = ok ? v[j + u * G] : 0.0;
+                const int jj = ok ? j + u * G : e - 1;
+                const int cv = ci[jj];
+                const double wv = v[jj];
+                c[u] = ok ? cv : r;
+                w[u] = ok ? wv : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -726,40 +731,61 @@ __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ pa
 
 // ---------------------------------------------------------------- levels
 
-// One dependency level of a lower-form triangular solve: rows
-// rows[r0 .. r0+cnt) depend only on rows of earlier levels, so G lanes per
-// row compute x[row] = (b[row] - sum_j a_rj x[j]) / a_rr in parallel, reading
-// earlier x from global memory (the previous launch has completed).  Values
-// and accumulation in T (FP64, or FP32 for the refined mode); b and x are
-// FP64 with the caller's index order (flip: n-1-i).
+// One dependency level of a lower-form triangular solve: its rows depend only
+// on rows of earlier levels, so G lanes per row compute x[row] = (b[row] -
+// sum_j a_rj x[j]) / a_rr in parallel, reading earlier x from global memory
+// (the previous launch has completed).  Values and accumulation in T (FP64,
+// or FP32 for the refined mode); b and x are FP64 in the caller's index order
+// (flip: n-1-i), NR right-hand sides.  Each lane sums its entries (first +
+// lane, stride G) in ascending order, then the group sum runs in a fixed
+// order.  Row records and entries are stored in level order, so a lane issues
+// its record load, then all of its column and value loads (U per trip, the
+// last trip predicated), then all of its gathers.
 template <int G, int NR, class T>
-__global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int *__restrict__ rows,
-                                               const int *__restrict__ ia, const int *__restrict__ ja,
-                                               const T *__restrict__ val, const T *__restrict__ invd,
-                                               const double *__restrict__ b, int ldb, double *x, int ldx,
-                                               int nr, int n, int flip)
+__global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int4 *__restrict__ rec,
+                                               const int *__restrict__ lja, const T *__restrict__ lval,
+                                               const T *__restrict__ invd, const double *__restrict__ b, int ldb,
+                                               double *x, int ldx, int nr, int n, int flip)
 {
+    constexpr int U = 4;
     const int gid = blockIdx.x * 256 + threadIdx.x;
     const int q = gid / G, li = gid % G;
-    const bool ok = q < cnt;
-    const int row = ok ? rows[r0 + q] : 0;
+    const int4 rc = q < cnt ? rec[r0 + q] : make_int4(-1, 0, 0, 0);
+    const bool ok = rc.x >= 0;   // padding records have row -1 and no entries
+    const int row = rc.x;
+    const int gm = flip ? n - 1 - row : row;
     T acc[NR];
 #pragma unroll
     for (int t = 0; t < NR; t++) acc[t] = (T)0;
-    if (ok) {
-        const int e1 = ia[row + 1] - 1;   // the diagonal is last
-        for (int e = ia[row] + li; e < e1; e += G) {
-            const T v = val[e];
-            const int c = ja[e], cm = flip ? n - 1 - c : c;
+    for (int e = rc.y + li; e < rc.z; e += U * G) {
+        int c[U];
+        T w[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // unconditional loads at a clamped index, masked entries then
+            // use the row itself (not used)
+            const bool in = e + u * G < rc.z;
+            const int ee = in ? e + u * G : rc.z - 1;
+            const int cl = lja[ee];
+            const T wl = lval[ee];
+            const int cc = in ? cl : row;
+            c[u] = flip ? n - 1 - cc : cc;
+            w[u] = in ? wl : (T)0;
+        }
+        T xv[U][NR];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int t = 0; t < NR; t++) xv[u][t] = t < nr ? (T)x[(size_t)t * ldx + c[u]] : (T)0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
 #pragma unroll
             for (int t = 0; t < NR; t++)
-                if (t < nr) acc[t] = fma(v, (T)x[(size_t)t * ldx + cm], acc[t]);
-        }
+                if (e + u * G < rc.z) acc[t] = fma(w[u], xv[u][t], acc[t]);
     }
 #pragma unroll
     for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
     if (ok && li == 0) {
-        const int gm = flip ? n - 1 - row : row;
         const T d = invd[row];
 #pragma unroll
         for (int t = 0; t < NR; t++)
@@ -770,21 +796,158 @@ __global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int *__res
 // Multi-RHS level step on interleaved vectors (v[i*NRP + r]): NRP lanes per
 // row, one per right-hand side, each summing the row's entries in order
 // (the entry's value and column are the same for all lanes of the row; the
-// x loads of one entry are one contiguous NRP*8-byte segment).
+// x loads of one entry are one contiguous NRP*8-byte segment).  Record,
+// then entries (U per trip, the last trip predicated), then gathers, as in
+// k_level.
 template <int NRP, class T>
-__global__ __launch_bounds__(256) void k_level_il(int r0, int cnt, const int *__restrict__ rows,
-                                                  const int *__restrict__ ia, const int *__restrict__ ja,
-                                                  const T *__restrict__ val, const T *__restrict__ invd,
-                                                  const double *__restrict__ bi, double *xi)
+__global__ __launch_bounds__(256) void k_level_il(int r0, int cnt, const int4 *__restrict__ rec,
+                                                  const int *__restrict__ lja, const T *__restrict__ lval,
+                                                  const T *__restrict__ invd, const double *__restrict__ bi,
+                                                  double *xi)
 {
+    constexpr int U = 4;
     const int gid = blockIdx.x * 256 + threadIdx.x;
     const int q = gid / NRP, r = gid % NRP;
     if (q >= cnt) return;
-    const int row = rows[r0 + q];
-    const int e1 = ia[row + 1] - 1;   // the diagonal is last
+    const int4 rc = rec[r0 + q];
+    asm volatile("" ::"v"(rc.y), "v"(rc.z));   // one record load, not two around the branch
+    if (rc.x < 0) return;   // padding
+    const int row = rc.x;
+    const double bv = bi[(size_t)row * NRP + r];
+    const T d = invd[row];
     T acc = (T)0;
-    for (int e = ia[row]; e < e1; e++) acc = fma(val[e], (T)xi[(size_t)ja[e] * NRP + r], acc);
-    xi[(size_t)row * NRP + r] = (double)(((T)bi[(size_t)row * NRP + r] - acc) * invd[row]);
+    for (int e = rc.y; e < rc.z; e += U) {
+        int c[U];
+        T w[U];
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // unconditional loads at a clamped index, masked entries then
+            // use the row itself (not used)
+            const bool in = e + u < rc.z;
+            const int ee = in ? e + u : rc.z - 1;
+            const int cl = lja[ee];
+            const T wl = lval[ee];
+            c[u] = in ? cl : row;
+            w[u] = in ? wl : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) xv[u] = xi[(size_t)c[u] * NRP + r];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (e + u < rc.z) acc = fma(w[u], (T)xv[u], acc);
+    }
+    xi[(size_t)row * NRP + r] = (double)(((T)bv - acc) * d);
+}
+
+// ---------------------------------------------------------------- sync-free
+
+// Sync-free form of the level schedule: one launch solves every row.  Rows
+// are taken in level order (position q of the level-ordered records; every
+// level starts at a multiple of 64 positions, so a wavefront never holds rows
+// of two levels), so a row only waits for rows at smaller positions, whose
+// workgroups were dispatched before its own and are running or done.
+//
+// The solution vector itself carries readiness: before the launch it is
+// filled with kNotReady, a signalling-NaN pattern that arithmetic never
+// produces (results of NaN operands are quiet NaNs), and each x value is
+// published by one 64-bit store.  A row polls the x values it reads until
+// none is kNotReady and computes with the values it polled.  Loads and stores
+// are agent-scope atomics, performed at the agent's coherence point (the
+// XCDs' L2 caches are not coherent with each other).  Waits are bounded:
+// after kSpinLimit polls (about a second) a row gives up, raises the shared
+// abort word and writes a quiet NaN, so a broken dispatch-order assumption
+// shows as NaN results instead of a hang; once the abort word is set every
+// waiting row gives up at once.  (The limit is a launch argument so that a
+// test can force the give-up path with a tiny one.)
+constexpr unsigned long long kNotReady = 0x7FF4DEADBEEF0001ull;
+
+__device__ __forceinline__ double load_agent(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool not_ready(double v)
+{
+    return (unsigned long long)__double_as_longlong(v) == kNotReady;
+}
+
+// Polls v[u] = src[idx[u]] (u with in[u]) until none is kNotReady; false if
+// the wait gave up.
+template <int U>
+__device__ __forceinline__ bool poll_values(const double *src, const size_t (&idx)[U], const bool (&in)[U],
+                                            double (&v)[U], int spin_limit, int *abort_word)
+{
+    for (int spins = 0;; spins++) {
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = load_agent(&src[idx[u]]);   // all polls in flight together
+        bool ready = true;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u] && not_ready(v[u])) ready = false;
+        if (ready) return true;
+        if (spins >= spin_limit ||
+            ((spins & 63) == 63 && __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Interleaved right-hand sides (v[i*NRP + r]), NRP lanes per row as in
+// k_level_il and with its arithmetic (same bits); each lane polls its own
+// right-hand side's values.
+template <int NRP, class T>
+__global__ __launch_bounds__(256) void k_sf_il(int cnt, const int4 *__restrict__ rec, const int *__restrict__ lja,
+                                               const T *__restrict__ lval, const T *__restrict__ invd,
+                                               const double *__restrict__ bi, double *xi, int spin_limit,
+                                               int *abort_word)
+{
+    constexpr int U = 4;
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int q = gid / NRP, r = gid % NRP;
+    if (q >= cnt) return;
+    const int4 rc = rec[q];
+    asm volatile("" ::"v"(rc.y), "v"(rc.z));   // one record load, not two around the branch
+    if (rc.x < 0) return;   // padding
+    const int row = rc.x;
+    const double bv = bi[(size_t)row * NRP + r];
+    const T d = invd[row];
+    T acc = (T)0;
+    bool ok = true;
+    for (int e = rc.y; e < rc.z && ok; e += U) {
+        size_t idx[U];
+        bool in[U];
+        T w[U];
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // unconditional loads at a clamped index; a masked entry polls
+            // the last entry again and is not used
+            in[u] = e + u < rc.z;
+            const int ee = in[u] ? e + u : rc.z - 1;
+            idx[u] = (size_t)lja[ee] * NRP + r;
+            const T wl = lval[ee];
+            w[u] = in[u] ? wl : (T)0;
+        }
+        ok = poll_values<U>(xi, idx, in, xv, spin_limit, abort_word);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u]) acc = fma(w[u], (T)xv[u], acc);
+    }
+    store_agent(&xi[(size_t)row * NRP + r], ok ? (double)(((T)bv - acc) * d) : __builtin_nan(""));
+}
+
+// v[0 .. len) = kNotReady; clears the abort word.
+__global__ __launch_bounds__(256) void k_not_ready(size_t len, double *v, int *abort_word)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < len) v[i] = __longlong_as_double((long long)kNotReady);
+    if (i == 0) *abort_word = 0;
 }
 
 // Interleave / de-interleave between the caller's column-major layout
@@ -1032,8 +1195,8 @@ template <int G, int NR, class T>
 hipError_t launch_level_t(int r0, int cnt, const LevelArgs &a, hipStream_t s)
 {
     const dim3 grid((unsigned)(((size_t)cnt * G + 255) / 256)), block(256);
-    hipLaunchKernelGGL((k_level<G, NR, T>), grid, block, 0, s, r0, cnt, a.rows, a.ia, a.ja,
-                       (const T *)(a.fp32 ? (const void *)a.val32 : (const void *)a.val),
+    hipLaunchKernelGGL((k_level<G, NR, T>), grid, block, 0, s, r0, cnt, (const int4 *)a.rec, a.lja,
+                       (const T *)(a.fp32 ? (const void *)a.lval32 : (const void *)a.lval),
                        (const T *)(a.fp32 ? (const void *)a.invd32 : (const void *)a.invd), a.b, a.ldb, a.x, a.ldx,
                        a.nr, a.n, a.flip);
     return hipGetLastError();
@@ -1075,11 +1238,11 @@ hipError_t launch_level_il_n(int r0, int cnt, const LevelArgs &a, const double *
 {
     const dim3 grid((unsigned)(((size_t)cnt * NRP + 255) / 256)), block(256);
     if (a.fp32)
-        hipLaunchKernelGGL((k_level_il<NRP, float>), grid, block, 0, s, r0, cnt, a.rows, a.ia, a.ja, a.val32, a.invd32,
-                           bi, xi);
+        hipLaunchKernelGGL((k_level_il<NRP, float>), grid, block, 0, s, r0, cnt, (const int4 *)a.rec, a.lja,
+                           a.lval32, a.invd32, bi, xi);
     else
-        hipLaunchKernelGGL((k_level_il<NRP, double>), grid, block, 0, s, r0, cnt, a.rows, a.ia, a.ja, a.val, a.invd,
-                           bi, xi);
+        hipLaunchKernelGGL((k_level_il<NRP, double>), grid, block, 0, s, r0, cnt, (const int4 *)a.rec, a.lja, a.lval,
+                           a.invd, bi, xi);
     return hipGetLastError();
 }
 
@@ -1092,6 +1255,38 @@ hipError_t launch_level_il(int r0, int cnt, int nrp, const LevelArgs &a, const d
     case 8: return launch_level_il_n<8>(r0, cnt, a, bi, xi, s);
     case 16: return launch_level_il_n<16>(r0, cnt, a, bi, xi, s);
     case 32: return launch_level_il_n<32>(r0, cnt, a, bi, xi, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NRP>
+static hipError_t sf_il_n(int npos, const LevelArgs &a, const double *bi, double *xi, int *abort_word, hipStream_t s)
+{
+    const dim3 grid((unsigned)(((size_t)npos * NRP + 255) / 256)), block(256);
+    if (a.fp32)
+        hipLaunchKernelGGL((k_sf_il<NRP, float>), grid, block, 0, s, npos, (const int4 *)a.rec, a.lja, a.lval32,
+                           a.invd32, bi, xi, a.spin_limit, abort_word);
+    else
+        hipLaunchKernelGGL((k_sf_il<NRP, double>), grid, block, 0, s, npos, (const int4 *)a.rec, a.lja, a.lval,
+                           a.invd, bi, xi, a.spin_limit, abort_word);
+    return hipGetLastError();
+}
+
+hipError_t launch_sync_free(int npos, int n, int nrp, const LevelArgs &a, const double *bi, double *xi,
+                            int *abort_word, hipStream_t s)
+{
+    if (npos <= 0 || n <= 0) return hipSuccess;
+    // the vector the rows publish into starts as kNotReady
+    const size_t len = (size_t)n * nrp;
+    hipLaunchKernelGGL(k_not_ready, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, xi, abort_word);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (nrp) {
+    case 2: return sf_il_n<2>(npos, a, bi, xi, abort_word, s);
+    case 4: return sf_il_n<4>(npos, a, bi, xi, abort_word, s);
+    case 8: return sf_il_n<8>(npos, a, bi, xi, abort_word, s);
+    case 16: return sf_il_n<16>(npos, a, bi, xi, abort_word, s);
+    case 32: return sf_il_n<32>(npos, a, bi, xi, abort_word, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -146,3 +146,40 @@ def test_levels_lap3d_64_refined_multi_rhs(native):
     X = f.solve(B, nrhs=4)
     for j in range(4):
         assert relerr(X[j], orc.ldsol(m.a, m.ia, m.ja, B[j])) <= TOL
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_sync_free_matches_level_launches(native, monkeypatch, uplo, precision):
+    """Multi-RHS level solves run as one sync-free launch (rows poll the x
+    values they read) or as one launch per level: same arithmetic per row,
+    so the same bits, and both at the parity bar."""
+    m = gen.lap3d(32)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    B = gen.rhs(m.n, 8, seed=gen.SEED_BASE + 180)
+    monkeypatch.setenv("SPIKE_PSTRSV_SYNC_FREE", "0")
+    f0 = native.Factor(uplo, m.a, m.ia, m.ja, method="levels", precision=precision)
+    monkeypatch.setenv("SPIKE_PSTRSV_SYNC_FREE", "1")
+    f1 = native.Factor(uplo, m.a, m.ia, m.ja, method="levels", precision=precision)
+    X0 = f0.solve(B, nrhs=8)
+    X1 = f1.solve(B, nrhs=8)
+    assert np.array_equal(X0, X1)
+    for j in range(8):
+        assert relerr(X1[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])) <= TOL
+
+
+def test_sync_free_give_up_is_bounded(native, monkeypatch):
+    """With a poll limit of 0 (forced from the host) rows that find an
+    operand unpublished give up at once: the solve returns, the rows that
+    gave up and everything after them read NaN, and every finite value is
+    the exact result."""
+    m = gen.lap3d(32)
+    B = gen.rhs(m.n, 8, seed=gen.SEED_BASE + 181)
+    want = native.Factor("L", m.a, m.ia, m.ja, method="levels").solve(B, nrhs=8)
+    monkeypatch.setenv("SPIKE_PSTRSV_SPIN_LIMIT", "0")
+    f = native.Factor("L", m.a, m.ia, m.ja, method="levels")
+    X = f.solve(B, nrhs=8)
+    fin = np.isfinite(X)
+    assert not fin.all(), "the give-up path did not run"
+    assert np.array_equal(X[fin], want[fin])
