@@ -229,12 +229,11 @@ std::vector<float> to_f32(const std::vector<double> &v)
     return std::vector<float>(v.begin(), v.end());
 }
 
-// Step stream to the device: FP64 values and multipliers, or (fp32) their
-// FP32 roundings only.
+// Step stream to the device: FP64 values, or (fp32) their FP32 roundings only.
 int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps &ds, bool fp32)
 {
-    double *val = nullptr, *mult = nullptr, *late = nullptr;
-    float *val32 = nullptr, *mult32 = nullptr;
+    double *val = nullptr, *late = nullptr;
+    float *val32 = nullptr;
     uint16_t *col, *row;
     int64_t *pb;
     int rc;
@@ -247,8 +246,8 @@ int upload_steps(spike_factor_s *f, const spike::StepStream &ss, spike::DevSteps
         if ((rc = f->upload(&val, ss.val))) return rc;
         if (ss.fwd && (rc = f->upload(&late, ss.late))) return rc;
     }
-    ds.val = val; ds.col = col; ds.row = row; ds.mult = mult; ds.late = late; ds.part_begin = pb;
-    ds.val32 = val32; ds.mult32 = mult32;
+    ds.val = val; ds.col = col; ds.row = row; ds.late = late; ds.part_begin = pb;
+    ds.val32 = val32;
     ds.G = ss.G; ds.M = ss.M; ds.R = ss.R; ds.fwd = ss.fwd ? 1 : 0;
     return SPIKE_OK;
 }
@@ -939,9 +938,9 @@ std::vector<std::pair<const char *, void **>> device_fields(spike_factor_s *f)
             {"ccol", (void **)&f->d_cpl_col},        {"cval", (void **)&f->d_cpl_val},
             {"sval", (void **)&f->steps.val},
             {"scol", (void **)&f->steps.col},        {"srow", (void **)&f->steps.row},
-            {"smul", (void **)&f->steps.mult},       {"slat", (void **)&f->steps.late},
+            {"slat", (void **)&f->steps.late},
             {"spb_", (void **)&f->steps.part_begin}, {"sv32", (void **)&f->steps.val32},
-            {"sm32", (void **)&f->steps.mult32},     {"invd", (void **)&f->d_inv_diag},
+            {"invd", (void **)&f->d_inv_diag},
             {"rrp_", (void **)&f->d_rrp},            {"rci_", (void **)&f->d_rci},
             {"rv__", (void **)&f->d_rv},             {"rlst", (void **)&f->d_rlist},
             {"rlev", (void **)&f->d_levptr},         {"swrw", (void **)&f->d_sweep_rows},

@@ -33,10 +33,8 @@ struct DevSteps {
     const double *val;
     const uint16_t *col;
     const uint16_t *row;
-    const double *mult;          // not fwd
     const double *late;          // fwd: R x R block per step
-    const float *val32;          // FP32 factors: values and multipliers
-    const float *mult32;
+    const float *val32;          // FP32 factors: values
     const int64_t *part_begin;   // P+1
     int G, M, R;
     int fwd;

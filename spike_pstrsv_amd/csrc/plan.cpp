@@ -67,8 +67,7 @@ void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
                         const std::vector<int> &ecol, const std::vector<double> &evals,
                         const std::vector<double> &diag, int G, int M, bool fwd,
                         std::vector<double> &val, std::vector<uint16_t> &col,
-                        std::vector<uint16_t> &row, std::vector<double> &mult,
-                        std::vector<double> &late, int &nsteps)
+                        std::vector<uint16_t> &row, std::vector<double> &late, int &nsteps)
 {
     const int R = 64 / G, cap = G * M;
     std::vector<Task> tasks;
@@ -138,7 +137,6 @@ void schedule_partition(int k, int lo, int hi, const std::vector<int> &rb,
         // values pre-scaled by the row's 1/diag (the right-hand side is
         // scaled on load), so every chunk of a row computes x -= sum val * x
         row.assign((size_t)nsteps * R, (uint16_t)(k + 1));
-        mult.clear();
         for (int s = 0; s < nsteps; s++) {
             for (int slot = 0; slot < (int)steps[s].size(); slot++) {
                 const Task &t = tasks[steps[s][slot]];
@@ -391,7 +389,7 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     ss.G = G; ss.M = M; ss.R = 64 / G; ss.fwd = fwd;
     pl.inv_diag.resize(n);
     for (int i = 0; i < n; i++) pl.inv_diag[i] = 1.0 / LV[LA[i + 1] - 1];
-    std::vector<std::vector<double>> pv(P), pm(P), pl_late(P);
+    std::vector<std::vector<double>> pv(P), pl_late(P);
     std::vector<std::vector<uint16_t>> pc(P), pr(P);
     std::vector<int> pn(P, 0);
     parallel_for(P, [&](int t) {
@@ -405,7 +403,7 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
             rb[r + 1] = (int)ecol.size();
             dg[r] = LV[LA[i + 1] - 1];
         }
-        schedule_partition(pt.k, 0, pt.k - 1, rb, ecol, ev, dg, G, M, fwd, pv[t], pc[t], pr[t], pm[t], pl_late[t],
+        schedule_partition(pt.k, 0, pt.k - 1, rb, ecol, ev, dg, G, M, fwd, pv[t], pc[t], pr[t], pl_late[t],
                            pn[t]);
     });
     ss.part_begin.assign(P + 1, 0);
@@ -418,7 +416,6 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     ss.col.resize((size_t)ss.nsteps * M * 64);
     ss.row.resize((size_t)ss.nsteps * ss.R);
     if (fwd) ss.late.resize((size_t)ss.nsteps * ss.R * ss.R);
-    else ss.mult.clear();
     parallel_for(P, [&](int t) {
         size_t s0 = (size_t)ss.part_begin[t];
         std::copy(pv[t].begin(), pv[t].end(), ss.val.begin() + s0 * M * 64);

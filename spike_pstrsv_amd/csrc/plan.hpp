@@ -48,7 +48,6 @@ struct StepStream {
     std::vector<double> val;            // nsteps * M * 64  [step][round][lane]
     std::vector<uint16_t> col;          // nsteps * M * 64  local column, pad = k_p
     std::vector<uint16_t> row;          // nsteps * R       local row, k+1 = empty
-    std::vector<double> mult;           // unused (values are pre-scaled)
     std::vector<double> late;           // nsteps * R * R   fwd only
 };
 
