@@ -169,6 +169,32 @@ __host__ __device__ __forceinline__ size_t xs_bytes(int kmax, int NR)
     return ((size_t)(kmax + XS_PAD) * NR * sizeof(T) + 7) & ~(size_t)7;
 }
 
+// xs[i*NR + r] = b_r[s+i] (times 1/diag when scaled) for i < k.  Each lane
+// takes UB rows per trip at clamped indices, so all loads of a trip are in
+// flight together (one memory round trip per trip, not per row).
+template <int NR, class T>
+__device__ __forceinline__ void load_rhs(const BlockArgs &A, const RhsMap &m, T *xs, int s, int k, int roff,
+                                         bool scaled)
+{
+    constexpr int UB = NR >= 8 ? 1 : 8 / NR;   // 8 values of b per lane and trip
+    const int lane = threadIdx.x;
+    for (int i0 = lane; i0 < k; i0 += 64 * UB) {
+        T sc[UB], bv[UB][NR];
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int i = min(i0 + 64 * u, k - 1);
+            sc[u] = scaled ? (T)A.inv_diag[s + i] : (T)1;
+#pragma unroll
+            for (int r = 0; r < NR; r++) bv[u][r] = (T)m.bv(roff + r, s + i);
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++)
+            if (i0 + 64 * u < k)
+#pragma unroll
+                for (int r = 0; r < NR; r++) xs[(i0 + 64 * u) * NR + r] = bv[u][r] * sc[u];
+    }
+}
+
 // Pass-2 right-hand side of one partition in LDS, NR right-hand sides
 // interleaved (xs[i*NR + r]): b for every row, then b - sum_j c_ij xr[col_j]
 // on the coupling rows (times 1/diag when scaled).  The coupling update is
@@ -183,22 +209,32 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k;
     const RhsMap m = rhs_map(A, pt);
-    for (int i = lane; i < k; i += 64) {
-        const int g = s + i;
-        const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
-#pragma unroll
-        for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, g) * sc;
-    }
+    load_rhs<NR, T>(A, m, xs, s, k, roff, scaled);
     if (pt.ccnt == 0) return;
     if (A.cpl_fused) {
         const double *cv = A.cpl_val;
         const int e0 = A.cpl_ptr[pt.cbeg], e1 = A.cpl_ptr[pt.cbeg + pt.ccnt];
-#pragma unroll 4
-        for (int e = e0 + lane; e < e1; e += 64) {
-            const double w = cv[e];
-            const double *xc = A.xr + (size_t)A.cpl_col[e] * RS + roff;
+        // UC entries per lane per trip at clamped indices: values and
+        // columns, then the gathers, then the products
+        constexpr int UC = NR >= 8 ? 1 : 8 / NR;
+        for (int e = e0 + lane; e < e1; e += 64 * UC) {
+            double w[UC], xv[UC][NR];
+            int col[UC];
 #pragma unroll
-            for (int r = 0; r < NR; r++) prod[(e - e0) * NR + r] = w * xc[r];
+            for (int u = 0; u < UC; u++) {
+                const int ee = min(e + 64 * u, e1 - 1);
+                w[u] = cv[ee];
+                col[u] = A.cpl_col[ee];
+            }
+#pragma unroll
+            for (int u = 0; u < UC; u++)
+#pragma unroll
+                for (int r = 0; r < NR; r++) xv[u][r] = A.xr[(size_t)col[u] * RS + roff + r];
+#pragma unroll
+            for (int u = 0; u < UC; u++)
+                if (e + 64 * u < e1)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) prod[(e + 64 * u - e0) * NR + r] = w[u] * xv[u][r];
         }
         __syncthreads();
         for (int c = lane; c < pt.ccnt; c += 64) {
@@ -250,11 +286,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
-        for (int i = lane; i < k; i += 64) {
-            const T sc = (T)A.inv_diag[s + i];
-#pragma unroll
-            for (int r = 0; r < NR; r++) xs[i * NR + r] = (T)m.bv(roff + r, s + i) * sc;
-        }
+        load_rhs<NR, T>(A, m, xs, s, k, roff, true);
     } else {
         pass2_rhs<NR, T>(A, pt, xs, reinterpret_cast<double *>(smem_blk + xs_bytes<T>(A.kmax, NR)), true, roff, RS);
     }
@@ -358,19 +390,43 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 epilogue:
     __syncthreads();
 
+    // Both epilogue loops take UE items per lane per trip, their loads
+    // (dependent-row indices; old x when accumulating) issued together.
+    constexpr int UE = NR >= 4 ? 1 : 4 / NR;
     if (PASS == 1)
-        for (int q = lane; q < pt.dcnt; q += 64) {
-            const int i = A.dep_rows[pt.dbeg + q] - s;
+        for (int q0 = lane; q0 < pt.dcnt; q0 += 64 * UE) {
+            int ir[UE];
 #pragma unroll
-            for (int r = 0; r < NR; r++) A.xred[(size_t)(pt.dbeg + q) * RS + roff + r] = (double)xs[i * NR + r];
+            for (int u = 0; u < UE; u++) ir[u] = A.dep_rows[pt.dbeg + min(q0 + 64 * u, pt.dcnt - 1)] - s;
+#pragma unroll
+            for (int u = 0; u < UE; u++)
+                if (q0 + 64 * u < pt.dcnt)
+#pragma unroll
+                    for (int r = 0; r < NR; r++)
+                        A.xred[(size_t)(pt.dbeg + q0 + 64 * u) * RS + roff + r] = (double)xs[ir[u] * NR + r];
         }
     // pass 2 writes the solution; without coupling, pass 1's result is final
     if (PASS == 2 || pt.f >= k)
-        for (int i = lane; i < k; i += 64) {
-            const int gm = m.at(s + i);
+        for (int i0 = lane; i0 < k; i0 += 64 * UE) {
+            double old[UE][NR];
 #pragma unroll
-            for (int r = 0; r < NR; r++)
-                if (roff + r < m.nr) put_x(m.x, (size_t)(roff + r) * m.ldx + gm, (double)xs[i * NR + r], A.accum);
+            for (int u = 0; u < UE; u++) {
+                const int gm = m.at(s + min(i0 + 64 * u, k - 1));
+#pragma unroll
+                for (int r = 0; r < NR; r++)
+                    old[u][r] = A.accum && roff + r < m.nr ? m.x[(size_t)(roff + r) * m.ldx + gm] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < UE; u++)
+                if (i0 + 64 * u < k) {
+                    const int gm = m.at(s + i0 + 64 * u);
+#pragma unroll
+                    for (int r = 0; r < NR; r++)
+                        if (roff + r < m.nr) {
+                            const double v = (double)xs[(i0 + 64 * u) * NR + r];
+                            m.x[(size_t)(roff + r) * m.ldx + gm] = A.accum ? old[u][r] + v : v;
+                        }
+                }
         }
 }
 
