@@ -153,7 +153,7 @@ struct spike_factor_s {
     // interleaved right-hand sides / solutions of multi-RHS level solves
     double *d_bi = nullptr, *d_xi = nullptr;
     size_t il_cap = 0;
-    // row blocks of the residual kernel (the partitions, or 1024-row blocks)
+    // row blocks of the residual kernel (at most 64 rows of one partition)
     int nres = 0;
     DevPart *d_res = nullptr;
 
@@ -448,7 +448,6 @@ int upload_row_blocks(spike_factor_s *f, const std::vector<DevPart> &parts)
     return f->upload(&f->d_res, rb);
 }
 
-// Device data of the level-scheduled method.
 // Sync-free switch (SPIKE_PSTRSV_SYNC_FREE=0 selects one launch per level)
 // and the abort word it needs (every solve clears it with a kernel).
 int setup_sync_free(spike_factor_s *f)
@@ -464,6 +463,7 @@ int setup_sync_free(spike_factor_s *f)
     return SPIKE_OK;
 }
 
+// Device data of the level-scheduled method.
 int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
 {
     Plan &pl = f->plan;

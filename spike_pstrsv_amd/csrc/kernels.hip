@@ -723,16 +723,33 @@ __global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ pa
 #pragma clang fp contract(off)
     const DevPart pt = parts[blockIdx.x];
     const RhsMap m = rhs_map(A, pt);
+    // U entries per trip at clamped indices: values and columns, then the
+    // gathers, then the sums in entry order
+    constexpr int U = NR >= 8 ? 2 : 8 / NR;
     for (int c = pt.cbeg + threadIdx.x; c < pt.cbeg + pt.ccnt; c += blockDim.x) {
         const int g = rows[c];
+        const int j0 = ptr[c], j1 = ptr[c + 1];
         double acc[NR];
 #pragma unroll
         for (int t = 0; t < NR; t++) acc[t] = 0.0;
-        for (int j = ptr[c]; j < ptr[c + 1]; j++) {
-            const double w = val[j];
-            const size_t x0 = (size_t)col[j] * NR;
+        for (int j = j0; j < j1; j += U) {
+            double w[U], xv[U][NR];
+            size_t x0[U];
 #pragma unroll
-            for (int t = 0; t < NR; t++) acc[t] += w * xred[x0 + t];
+            for (int u = 0; u < U; u++) {
+                const int jj = min(j + u, j1 - 1);
+                w[u] = val[jj];
+                x0[u] = (size_t)col[jj] * NR;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int t = 0; t < NR; t++) xv[u][t] = xred[x0[u] + t];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j + u < j1)
+#pragma unroll
+                    for (int t = 0; t < NR; t++) acc[t] += w[u] * xv[u][t];
         }
 #pragma unroll
         for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = m.bv(t, g) - acc[t];
@@ -741,17 +758,19 @@ __global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ pa
 
 // ---------------------------------------------------------------- residual
 
-// r = b - A x in FP64, one workgroup per partition, G = 8 lanes per
-// lower-form row (strided over its entries), NR right-hand sides (nr of the
-// partition's matrix active).  r uses the same caller addressing as b (ldr
-// replaces ldb when the partition has no leading dimension of its own).
-// Lane sums are combined with DPP in a fixed order.
+// r = b - A x in FP64, one workgroup per block of at most 64 lower-form rows
+// (the row blocks of a factor), G = 4 lanes per row (strided over its
+// entries), NR right-hand sides (nr of the block's matrix active).  r uses
+// the same caller addressing as b (ldr replaces ldb when the block has no
+// leading dimension of its own).  Each lane loads U entries per trip at
+// clamped indices, then their x values, then sums them in entry order; lane
+// sums are combined with DPP in a fixed order.
 template <int NR>
 __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ parts, const int *__restrict__ ia,
                                                   const int *__restrict__ ja, const double *__restrict__ a,
                                                   BlockArgs A, double *__restrict__ r, int ldr, int c0)
 {
-    constexpr int G = 8;
+    constexpr int G = 4, U = NR >= 8 ? 2 : (NR >= 4 ? 4 : 8);
     const DevPart pt = parts[blockIdx.x];
     const RhsMap m = rhs_map(A, pt);
     const int ld_r = pt.ld ? pt.ld : ldr;
@@ -765,12 +784,25 @@ __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ pa
         for (int t = 0; t < NR; t++) acc[t] = 0.0;
         if (lr < pt.k) {
             const int e1 = ia[row + 1];
-            for (int e = ia[row] + li; e < e1; e += G) {
-                const double v = a[e];
-                const int cm = m.at(ja[e]);
+            for (int e = ia[row] + li; e < e1; e += U * G) {
+                double v[U], xv[U][NR];
+                int cm[U];
 #pragma unroll
-                for (int t = 0; t < NR; t++)
-                    if (c0 + t < m.nr) acc[t] = fma(v, m.x[(size_t)(c0 + t) * m.ldx + cm], acc[t]);
+                for (int u = 0; u < U; u++) {
+                    const int ee = min(e + u * G, e1 - 1);
+                    v[u] = a[ee];
+                    cm[u] = m.at(ja[ee]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int t = 0; t < NR; t++)
+                        xv[u][t] = c0 + t < m.nr ? m.x[(size_t)(c0 + t) * m.ldx + cm[u]] : 0.0;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (e + u * G < e1)
+#pragma unroll
+                        for (int t = 0; t < NR; t++) acc[t] = fma(v[u], xv[u][t], acc[t]);
             }
         }
 #pragma unroll
