@@ -187,6 +187,14 @@ __device__ __forceinline__ void load_rhs(const BlockArgs &A, const RhsMap &m, T 
 #pragma unroll
             for (int r = 0; r < NR; r++) bv[u][r] = (T)m.bv(roff + r, s + i);
         }
+        // every value of the trip is used here, so no load sinks into the
+        // conditional stores below (which would wait per row again)
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            asm volatile("" ::"v"(sc[u]));
+#pragma unroll
+            for (int r = 0; r < NR; r++) asm volatile("" ::"v"(bv[u][r]));
+        }
 #pragma unroll
         for (int u = 0; u < UB; u++)
             if (i0 + 64 * u < k)
@@ -230,6 +238,12 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
             for (int u = 0; u < UC; u++)
 #pragma unroll
                 for (int r = 0; r < NR; r++) xv[u][r] = A.xr[(size_t)col[u] * RS + roff + r];
+#pragma unroll
+            for (int u = 0; u < UC; u++) {   // all loads before the conditional stores
+                asm volatile("" ::"v"(w[u]));
+#pragma unroll
+                for (int r = 0; r < NR; r++) asm volatile("" ::"v"(xv[u][r]));
+            }
 #pragma unroll
             for (int u = 0; u < UC; u++)
                 if (e + 64 * u < e1)
@@ -397,7 +411,10 @@ epilogue:
         for (int q0 = lane; q0 < pt.dcnt; q0 += 64 * UE) {
             int ir[UE];
 #pragma unroll
-            for (int u = 0; u < UE; u++) ir[u] = A.dep_rows[pt.dbeg + min(q0 + 64 * u, pt.dcnt - 1)] - s;
+            for (int u = 0; u < UE; u++) {
+                ir[u] = A.dep_rows[pt.dbeg + min(q0 + 64 * u, pt.dcnt - 1)] - s;
+                asm volatile("" ::"v"(ir[u]));   // all loads before the conditional stores
+            }
 #pragma unroll
             for (int u = 0; u < UE; u++)
                 if (q0 + 64 * u < pt.dcnt)
@@ -413,8 +430,10 @@ epilogue:
             for (int u = 0; u < UE; u++) {
                 const int gm = m.at(s + min(i0 + 64 * u, k - 1));
 #pragma unroll
-                for (int r = 0; r < NR; r++)
+                for (int r = 0; r < NR; r++) {
                     old[u][r] = A.accum && roff + r < m.nr ? m.x[(size_t)(roff + r) * m.ldx + gm] : 0.0;
+                    asm volatile("" ::"v"(old[u][r]));   // all loads before the conditional stores
+                }
             }
 #pragma unroll
             for (int u = 0; u < UE; u++)
