@@ -137,7 +137,7 @@ enum {
                               /* workgroup                                     */
     SPIKE_REDUCED_JACOBI = 2  /* S Jacobi sweeps x = y - W x over the whole    */
                               /* GPU; S is fixed at preprocessing so that the  */
-                              /* bound sum_{j>S} max(|W|^j 1) <= 1e-15 holds   */
+                              /* bound sum_{j>S} max(|W|^j 1) <= 2e-15 holds   */
 };
 
 /* Fills opts with the defaults. */

