@@ -217,11 +217,42 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k;
     const RhsMap m = rhs_map(A, pt);
+    if (pt.ccnt == 0) {
+        load_rhs<NR, T>(A, m, xs, s, k, roff, scaled);
+        return;
+    }
+    // The coupling rows of this lane (up to UR of them when the partition has
+    // at most 64 * UR; otherwise a loop after the barrier): their indices,
+    // right-hand sides and 1/diag are loaded first, so that these loads, the
+    // plain right-hand sides and the coupling products overlap instead of
+    // following each other.
+    constexpr int UR = 2;
+    const bool few = pt.ccnt <= 64 * UR;
+    int gr[UR], pa[UR], pb[UR];
+#pragma unroll
+    for (int u = 0; u < UR; u++) {
+        const int cc = pt.cbeg + min(lane + 64 * u, pt.ccnt - 1);
+        gr[u] = A.cpl_rows[cc];
+        pa[u] = A.cpl_fused ? A.cpl_ptr[cc] : 0;
+        pb[u] = A.cpl_fused ? A.cpl_ptr[cc + 1] : 0;
+    }
+    const int e0 = A.cpl_fused ? A.cpl_ptr[pt.cbeg] : 0;
+    const int e1 = A.cpl_fused ? A.cpl_ptr[pt.cbeg + pt.ccnt] : 0;
     load_rhs<NR, T>(A, m, xs, s, k, roff, scaled);
-    if (pt.ccnt == 0) return;
+    T scr[UR];
+    double bvr[UR][NR];
+    if (few) {
+#pragma unroll
+        for (int u = 0; u < UR; u++) {
+            scr[u] = scaled ? (T)A.inv_diag[gr[u]] : (T)1;
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+                bvr[u][r] = A.cpl_fused ? m.bv(roff + r, gr[u])
+                                        : A.bc[(size_t)(pt.cbeg + min(lane + 64 * u, pt.ccnt - 1)) * RS + roff + r];
+        }
+    }
     if (A.cpl_fused) {
         const double *cv = A.cpl_val;
-        const int e0 = A.cpl_ptr[pt.cbeg], e1 = A.cpl_ptr[pt.cbeg + pt.ccnt];
         // UC entries per lane per trip at clamped indices: values and
         // columns, then the gathers, then the products
         constexpr int UC = NR >= 8 ? 1 : 8 / NR;
@@ -250,10 +281,28 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
 #pragma unroll
                     for (int r = 0; r < NR; r++) prod[(e + 64 * u - e0) * NR + r] = w[u] * xv[u][r];
         }
-        __syncthreads();
-        for (int c = lane; c < pt.ccnt; c += 64) {
-            const int g = A.cpl_rows[pt.cbeg + c];
-            const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
+    }
+    __syncthreads();
+    if (few) {
+#pragma unroll
+        for (int u = 0; u < UR; u++)
+            if (lane + 64 * u < pt.ccnt) {
+                double acc[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++) acc[r] = 0.0;
+                for (int e = pa[u]; e < pb[u]; e++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
+#pragma unroll
+                for (int r = 0; r < NR; r++)
+                    xs[(gr[u] - s) * NR + r] = A.cpl_fused ? (T)(bvr[u][r] - acc[r]) * scr[u] : (T)bvr[u][r] * scr[u];
+            }
+        return;
+    }
+    for (int c = lane; c < pt.ccnt; c += 64) {
+        const int g = A.cpl_rows[pt.cbeg + c];
+        const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
+        if (A.cpl_fused) {
             double acc[NR];
 #pragma unroll
             for (int r = 0; r < NR; r++) acc[r] = 0.0;
@@ -262,12 +311,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
                 for (int r = 0; r < NR; r++) acc[r] += prod[(e - e0) * NR + r];
 #pragma unroll
             for (int r = 0; r < NR; r++) xs[(g - s) * NR + r] = (T)(m.bv(roff + r, g) - acc[r]) * sc;
-        }
-    } else {
-        __syncthreads();
-        for (int c = lane; c < pt.ccnt; c += 64) {
-            const int g = A.cpl_rows[pt.cbeg + c];
-            const T sc = scaled ? (T)A.inv_diag[g] : (T)1;
+        } else {
 #pragma unroll
             for (int r = 0; r < NR; r++)
                 xs[(g - s) * NR + r] = (T)A.bc[(size_t)(pt.cbeg + c) * RS + roff + r] * sc;
