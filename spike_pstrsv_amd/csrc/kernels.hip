@@ -342,6 +342,56 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const RhsMap m = rhs_map(A, pt);
     const int roff = blockIdx.y * NR, RS = A.nr_slots;
 
+    constexpr unsigned TB = sizeof(T);
+    const int64_t sb = A.st.part_begin[p];
+    const int nst = (int)(A.st.part_begin[p + 1] - sb);
+    const unsigned nstu = (unsigned)(nst > 0 ? nst : 1);
+    const __amdgpu_buffer_rsrc_t bval = make_rsrc(VT<T>::val(A) + (size_t)sb * M * 64, nstu * M * 64u * TB);
+    const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
+    const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
+    // Register ring of D step records (slot d holds step base + d).  With
+    // one RHS per wave and several lanes per row the first D records are
+    // loaded before the right-hand sides, so their latency overlaps with that
+    // prologue (measured 4 us per solve faster on a 1M-row banded matrix at
+    // 4 lanes per row).  Otherwise (more RHS per wave, or one lane per row)
+    // that raised the register count and slowed the step loop (2 RHS per wave
+    // on the banded matrix: 77 us per pass against 58; a 2D Laplacian at one
+    // lane per row: 87 against 78), so there the ring starts as empty padding
+    // steps (column k, row k+1) and the loop starts D steps early, its first
+    // turn only issuing loads.  The loop runs whole
+    // ring turns (the last < D steps run in the unrolled tail below), each
+    // step refilling its slot with the record D steps ahead (clamped to the
+    // last step), so loads stay D steps in flight.
+    // Columns and rows stay 16-bit in the ring: widening them right after the
+    // load would make the compiler wait for the load at the loop back-edge.
+    T rv[D][M];
+    uint16_t rc[D][M];
+    uint16_t rrow[D];
+    auto load_step = [&](int d, unsigned st) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            rv[d][m] = VT<T>::load(bval, (unsigned)(m * 64 + lane) * TB, st * (unsigned)(M * 64) * TB);
+            rc[d][m] = buf_u16(bcol, (unsigned)(m * 64 + lane) * 2u, st * (unsigned)(M * 128));
+        }
+        rrow[d] = buf_u16(brow, (unsigned)slot * 2u, st * (unsigned)(R * 2));
+    };
+    constexpr bool early = NR == 1 && G > 1;
+    if constexpr (early) {
+        if (nst > 0)
+#pragma unroll
+            for (int d = 0; d < D; d++) load_step(d, (unsigned)(d < nst ? d : nst - 1));
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                rv[d][m] = (T)0;
+                rc[d][m] = (uint16_t)k;
+            }
+            rrow[d] = (uint16_t)(k + 1);
+        }
+    }
+
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
         load_rhs<NR, T>(A, m, xs, s, k, roff, true);
@@ -352,36 +402,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 #pragma unroll
         for (int r = 0; r < NR; r++) xs[k * NR + r] = (T)0;
     __syncthreads();
-
-    constexpr unsigned TB = sizeof(T);
-    const int64_t sb = A.st.part_begin[p];
-    const int nst = (int)(A.st.part_begin[p + 1] - sb);
-    const unsigned nstu = (unsigned)(nst > 0 ? nst : 1);
-    const __amdgpu_buffer_rsrc_t bval = make_rsrc(VT<T>::val(A) + (size_t)sb * M * 64, nstu * M * 64u * TB);
-    const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
-    const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
-    // Register ring of D step records.  It starts as empty padding steps
-    // (column k, no row) and the loop starts D steps early, so the first D
-    // iterations compute nothing and only issue loads.  Every load then comes
-    // from the same instructions in the same order, and the loop runs whole
-    // iterations only (the last < D steps run in the unrolled tail below), so
-    // the compiler's memory wait counters stay exact across iterations and
-    // loads stay D steps in flight.
-    // Columns and rows stay 16-bit in the ring: widening them right after the
-    // load would make the compiler wait for the load at the loop back-edge.
-    T rv[D][M];
-    uint16_t rc[D][M];
-    uint16_t rrow[D];
     if (nst <= 0) goto epilogue;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            rv[d][m] = (T)0;
-            rc[d][m] = (uint16_t)k;
-        }
-        rrow[d] = (uint16_t)(k + 1);
-    }
     {
         // One step from ring slot d: LDS reads first (operands and the row's
         // accumulator), then the dot products.  Empty row slots name slot
@@ -425,15 +446,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // Refill slot d with step sidx + D (clamped to the last step), after
         // the last use of the slot's registers.
         auto refill = [&](int d, int sidx) __attribute__((always_inline)) {
-            const unsigned nxt = (unsigned)(sidx + D < nst ? sidx + D : nst - 1);
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                rv[d][m] = VT<T>::load(bval, (unsigned)(m * 64 + lane) * TB, nxt * (unsigned)(M * 64) * TB);
-                rc[d][m] = buf_u16(bcol, (unsigned)(m * 64 + lane) * 2u, nxt * (unsigned)(M * 128));
-            }
-            rrow[d] = buf_u16(brow, (unsigned)slot * 2u, nxt * (unsigned)(R * 2));
+            load_step(d, (unsigned)(sidx + D < nst ? sidx + D : nst - 1));
         };
-        int base = -D;
+        int base = early ? 0 : -D;
         for (; base + D <= nst; base += D) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
