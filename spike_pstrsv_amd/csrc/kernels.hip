@@ -332,6 +332,10 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     constexpr int R = 64 / G;
     const int p = blockIdx.x;
     const DevPart pt = A.parts[p];
+    // the step range is loaded with the partition record (one round trip
+    // for both, not one after the other around the early returns)
+    const int64_t sb = A.st.part_begin[p], se = A.st.part_begin[p + 1];
+    asm volatile("" ::"s"(sb), "s"(se));
     const int lane = threadIdx.x;
     const int slot = lane / G, li = lane % G;
     const int s = pt.s, k = pt.k;
@@ -343,8 +347,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const int roff = blockIdx.y * NR, RS = A.nr_slots;
 
     constexpr unsigned TB = sizeof(T);
-    const int64_t sb = A.st.part_begin[p];
-    const int nst = (int)(A.st.part_begin[p + 1] - sb);
+    const int nst = (int)(se - sb);
     const unsigned nstu = (unsigned)(nst > 0 ? nst : 1);
     const __amdgpu_buffer_rsrc_t bval = make_rsrc(VT<T>::val(A) + (size_t)sb * M * 64, nstu * M * 64u * TB);
     const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
