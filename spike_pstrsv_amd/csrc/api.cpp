@@ -616,8 +616,9 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     std::vector<DevPart> hp(P);
     for (int t = 0, mi = 0; t < P; t++) {
         const spike::Part &pt = pl.parts[t];
+        const int ce0 = pt.ccnt > 0 ? pl.cpl_ptr[pt.cbeg] : 0, ce1 = pt.ccnt > 0 ? pl.cpl_ptr[pt.cbeg + pt.ccnt] : 0;
         hp[t] = DevPart{pt.s, pt.k, pt.f, pt.hi1, pt.dbeg, pt.dcnt, pt.cbeg, pt.ccnt,
-                        0, n, pl.flip ? 1 : 0, 0, 0, 0, 0};
+                        0, n, pl.flip ? 1 : 0, 0, 0, ce0, ce1, 0, 0};
         if (!f->mats.empty()) {
             while (mi + 1 < (int)f->mats.size() && f->mats[mi].row0 + f->mats[mi].n <= pt.s) mi++;
             const spike_factor_s::Mat &m = f->mats[mi];
@@ -927,7 +928,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 // File layout: magic, a block of scalars, then tagged records
 // {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
 // stored by content; workspaces only by size (re-allocated on load).
-constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '3'};
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '4'};
 
 struct FactorScalars {
     spike_options_t opts;

@@ -236,8 +236,7 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
         pa[u] = A.cpl_fused ? A.cpl_ptr[cc] : 0;
         pb[u] = A.cpl_fused ? A.cpl_ptr[cc + 1] : 0;
     }
-    const int e0 = A.cpl_fused ? A.cpl_ptr[pt.cbeg] : 0;
-    const int e1 = A.cpl_fused ? A.cpl_ptr[pt.cbeg + pt.ccnt] : 0;
+    const int e0 = pt.cpe0, e1 = pt.cpe1;
     load_rhs<NR, T>(A, m, xs, s, k, roff, scaled);
     T scr[UR];
     double bvr[UR][NR];

@@ -24,7 +24,9 @@ struct DevPart {
     // index (mflip ? mn-1-l : l); its right-hand sides and solutions start at
     // b + boff / x + boff with leading dimension ld and mnr columns (ld == 0 /
     // mnr == 0: the launch's ldb, ldx and nr).
-    int row0, mn, mflip, ld, mnr, pad_;
+    int row0, mn, mflip, ld, mnr;
+    int cpe0, cpe1; // coupling entries of the partition (cpl_ptr[cbeg], cpl_ptr[cbeg+ccnt])
+    int pad_;
     int64_t boff;
 };
 
