@@ -394,6 +394,19 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         }
     }
 
+    // Pass 1 with one RHS: the local rows of the first 64*UP dependents
+    // (written back in the epilogue) are loaded here, with the right-hand
+    // sides, so the epilogue issues no loads for them.  (Several RHS load
+    // them per trip in the epilogue: holding them across the step loop raised
+    // the register count of those variants.)
+    constexpr int UP = 8;
+    constexpr bool dep_early = PASS == 1 && NR == 1;
+    int ir0[UP];
+#pragma unroll
+    for (int u = 0; u < UP; u++) ir0[u] = 0;
+    if (dep_early && pt.dcnt > 0)
+#pragma unroll
+        for (int u = 0; u < UP; u++) ir0[u] = A.dep_rows[pt.dbeg + min(lane + 64 * u, pt.dcnt - 1)] - s;
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
         load_rhs<NR, T>(A, m, xs, s, k, roff, true);
@@ -465,49 +478,90 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 epilogue:
     __syncthreads();
 
-    // Both epilogue loops take UE items per lane per trip, their loads
-    // (dependent-row indices; old x when accumulating) issued together.
+    // The epilogue loops read UE (UW) items per lane and trip from LDS at
+    // clamped indices before their conditional stores.  Global loads are
+    // issued before any store: a load behind a store makes its wait cover
+    // that store too (one counter for both), i.e. a write round trip per trip.
     constexpr int UE = NR >= 4 ? 1 : 4 / NR;
-    if (PASS == 1)
-        for (int q0 = lane; q0 < pt.dcnt; q0 += 64 * UE) {
+    constexpr int UW = NR >= 8 ? 1 : 8 / NR;
+    if (PASS == 1) {
+        int q = 0;   // dependents written so far
+        if (dep_early) {
+            T v[UP];
+#pragma unroll
+            for (int u = 0; u < UP; u++) v[u] = xs[ir0[u]];
+#pragma unroll
+            for (int u = 0; u < UP; u++)
+                if (lane + 64 * u < pt.dcnt) A.xred[(size_t)(pt.dbeg + lane + 64 * u) * RS + roff] = (double)v[u];
+            q = 64 * UP;
+        }
+        // the rest (several RHS, or more dependents): indices loaded per trip
+        for (int b = q; b < pt.dcnt; b += 64 * UE) {
+            const int q0 = b + lane;
             int ir[UE];
 #pragma unroll
-            for (int u = 0; u < UE; u++) {
-                ir[u] = A.dep_rows[pt.dbeg + min(q0 + 64 * u, pt.dcnt - 1)] - s;
-                asm volatile("" ::"v"(ir[u]));   // all loads before the conditional stores
-            }
+            for (int u = 0; u < UE; u++) ir[u] = A.dep_rows[pt.dbeg + min(q0 + 64 * u, pt.dcnt - 1)] - s;
+#pragma unroll
+            for (int u = 0; u < UE; u++) asm volatile("" ::"v"(ir[u]));   // all loads before the stores
+            T v[UE][NR];
+#pragma unroll
+            for (int u = 0; u < UE; u++)
+#pragma unroll
+                for (int r = 0; r < NR; r++) v[u][r] = xs[ir[u] * NR + r];
 #pragma unroll
             for (int u = 0; u < UE; u++)
                 if (q0 + 64 * u < pt.dcnt)
 #pragma unroll
                     for (int r = 0; r < NR; r++)
-                        A.xred[(size_t)(pt.dbeg + q0 + 64 * u) * RS + roff + r] = (double)xs[ir[u] * NR + r];
+                        A.xred[(size_t)(pt.dbeg + q0 + 64 * u) * RS + roff + r] = (double)v[u][r];
         }
+    }
     // pass 2 writes the solution; without coupling, pass 1's result is final
-    if (PASS == 2 || pt.f >= k)
-        for (int i0 = lane; i0 < k; i0 += 64 * UE) {
-            double old[UE][NR];
+    if (PASS == 2 || pt.f >= k) {
+        if (!A.accum) {
+            // plain write-back: LDS reads and stores only
+            for (int i0 = lane; i0 < k; i0 += 64 * UW) {
+                T v[UW][NR];
 #pragma unroll
-            for (int u = 0; u < UE; u++) {
-                const int gm = m.at(s + min(i0 + 64 * u, k - 1));
+                for (int u = 0; u < UW; u++)
 #pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    old[u][r] = A.accum && roff + r < m.nr ? m.x[(size_t)(roff + r) * m.ldx + gm] : 0.0;
-                    asm volatile("" ::"v"(old[u][r]));   // all loads before the conditional stores
-                }
+                    for (int r = 0; r < NR; r++) v[u][r] = xs[min(i0 + 64 * u, k - 1) * NR + r];
+#pragma unroll
+                for (int u = 0; u < UW; u++)
+                    if (i0 + 64 * u < k) {
+                        const int gm = m.at(s + i0 + 64 * u);
+#pragma unroll
+                        for (int r = 0; r < NR; r++)
+                            if (roff + r < m.nr) m.x[(size_t)(roff + r) * m.ldx + gm] = (double)v[u][r];
+                    }
             }
+        } else {
+            // refinement update x += d: the old values of a trip are loaded
+            // together (columns beyond nr read column nr-1, unused)
+            for (int i0 = lane; i0 < k; i0 += 64 * UE) {
+                double old[UE][NR];
 #pragma unroll
-            for (int u = 0; u < UE; u++)
-                if (i0 + 64 * u < k) {
-                    const int gm = m.at(s + i0 + 64 * u);
+                for (int u = 0; u < UE; u++) {
+                    const int gm = m.at(s + min(i0 + 64 * u, k - 1));
 #pragma unroll
-                    for (int r = 0; r < NR; r++)
-                        if (roff + r < m.nr) {
-                            const double v = (double)xs[(i0 + 64 * u) * NR + r];
-                            m.x[(size_t)(roff + r) * m.ldx + gm] = A.accum ? old[u][r] + v : v;
-                        }
+                    for (int r = 0; r < NR; r++) old[u][r] = m.x[(size_t)min(roff + r, m.nr - 1) * m.ldx + gm];
                 }
+#pragma unroll
+                for (int u = 0; u < UE; u++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) asm volatile("" ::"v"(old[u][r]));   // all loads first
+#pragma unroll
+                for (int u = 0; u < UE; u++)
+                    if (i0 + 64 * u < k) {
+                        const int gm = m.at(s + i0 + 64 * u);
+#pragma unroll
+                        for (int r = 0; r < NR; r++)
+                            if (roff + r < m.nr)
+                                m.x[(size_t)(roff + r) * m.ldx + gm] = old[u][r] + (double)xs[(i0 + 64 * u) * NR + r];
+                    }
+            }
         }
+    }
 }
 
 // Forwarding block solve (16 lanes per row slot, 4 slots per step).  Values
