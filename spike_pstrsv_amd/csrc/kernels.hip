@@ -364,11 +364,12 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     // ring turns (the last < D steps run in the unrolled tail below), each
     // step refilling its slot with the record D steps ahead (clamped to the
     // last step), so loads stay D steps in flight.
-    // Columns and rows stay 16-bit in the ring: widening them right after the
-    // load would make the compiler wait for the load at the loop back-edge.
+    // Columns and rows are held 32-bit: the 16-bit buffer loads zero-extend,
+    // so no instruction widens them at their use (16-bit ring fields cost two
+    // masking instructions per step on the step chain).
     T rv[D][M];
-    uint16_t rc[D][M];
-    uint16_t rrow[D];
+    unsigned rc[D][M];
+    unsigned rrow[D];
     auto load_step = [&](int d, unsigned st) __attribute__((always_inline)) {
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -388,9 +389,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 rv[d][m] = (T)0;
-                rc[d][m] = (uint16_t)k;
+                rc[d][m] = (unsigned)k;
             }
-            rrow[d] = (uint16_t)(k + 1);
+            rrow[d] = (unsigned)(k + 1);
         }
     }
 
@@ -610,16 +611,16 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
         const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * 64, nstu * 128u);
         const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
         const __amdgpu_buffer_rsrc_t blate = make_rsrc(A.st.late + (size_t)sb * R * R, nstu * R * R * 8u);
-        // register ring (16-bit fields stay 16-bit, see k_block_solve)
+        // register ring (32-bit index fields, as in k_block_solve)
         double rv[D];
-        uint16_t rc[D];
-        uint16_t rrow[D];
+        unsigned rc[D];
+        unsigned rrow[D];
         double rl[D][R];
 #pragma unroll
         for (int d = 0; d < D; d++) {
             rv[d] = 0.0;
-            rc[d] = (uint16_t)k;
-            rrow[d] = (uint16_t)(k + 1);
+            rc[d] = (unsigned)k;
+            rrow[d] = (unsigned)(k + 1);
 #pragma unroll
             for (int t = 0; t < R; t++) rl[d][t] = 0.0;
         }
