@@ -423,7 +423,49 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // One step from ring slot d: LDS reads first (operands and the row's
         // accumulator), then the dot products.  Empty row slots name slot
         // k+1, so no lane tests for them.
+        //
+        // One RHS (the latency-bound case): lane 0 of the group starts its
+        // dot product from the row's accumulator, the other lanes from the
+        // zero slot k; lane 0 writes the row and every other lane its own
+        // spare slot k+2+lane, so neither needs a branch.  With several
+        // lanes per row both addresses select between the row's LDS address
+        // and an address fixed for the lane (opaque to the compiler, which
+        // would otherwise form each address after its select), and the
+        // addresses of step d+1 are formed during step d, after its LDS
+        // reads are issued, off the path from step d's write to step d+1's
+        // reads; the empty asm statement keeps them there (the scheduler
+        // otherwise hoisted them to the start of a ring turn, where they
+        // waited for ring loads issued only a few steps before).  Measured on
+        // a 1M-row banded matrix at 4 lanes per row: 44.1 / 47.6 us per pass
+        // against 44.7 / 48.8.
+        using LdsT = __attribute__((address_space(3))) T;
+        LdsT *const xl = (LdsT *)xs;
+        LdsT *zero_p = xl + k, *spare_p = xl + (k + 2 + lane);
+        asm volatile("" : "+v"(zero_p), "+v"(spare_p));
+        LdsT *a_col = xl, *a_acc = xl, *a_wr = xl;
+        auto addr = [&](int d) __attribute__((always_inline)) {
+            unsigned row = rrow[d], col = rc[d][0];
+            asm volatile("" : "+v"(row), "+v"(col));
+            LdsT *const rp = xl + row;
+            a_col = xl + col;
+            a_acc = li == 0 ? rp : zero_p;
+            a_wr = li == 0 ? rp : spare_p;
+        };
+        // (one lane per row keeps forming its addresses within the step:
+        // a 2D Laplacian's pass 2 measured 80 us piped against 74)
+        constexpr bool piped = NR == 1 && M == 1 && G > 1;
+        if constexpr (piped) addr(0);
         auto step = [&](int d) __attribute__((always_inline)) {
+            if constexpr (piped) {
+                const T xv = *a_col;
+                T t = *a_acc;
+                LdsT *const wr = a_wr;
+                addr((d + 1) % D);
+                t = fma(-rv[d][0], xv, t);
+                t = group_sum<G>(t);
+                *wr = t;
+                return;
+            }
             const unsigned row = rrow[d];
             T xv[M][NR], t[NR];
 #pragma unroll
@@ -431,11 +473,6 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
 #pragma unroll
                 for (int r = 0; r < NR; r++) xv[m][r] = xs[(unsigned)rc[d][m] * NR + r];
             if constexpr (NR == 1) {
-                // One RHS: lane 0 of the group starts its dot product from
-                // the row's accumulator, the other lanes from the zero slot k
-                // (the choice is an address, off the step chain); lane 0
-                // writes the row and every other lane its own spare slot
-                // k+2+lane, so the write needs no branch either.
                 t[0] = xs[li == 0 ? row : (unsigned)k];
 #pragma unroll
                 for (int m = 0; m < M; m++) t[0] = fma(-rv[d][m], xv[m][0], t[0]);
