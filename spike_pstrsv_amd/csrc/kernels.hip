@@ -318,6 +318,19 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
     }
 }
 
+// Value type of one block-solve LDS slot with NR interleaved right-hand
+// sides (NR = 1 or 2): the NR values of a slot are read and written as one.
+template <class T, int NR>
+struct SlotVec;
+template <class T>
+struct SlotVec<T, 1> {
+    using type = T;
+};
+template <class T>
+struct SlotVec<T, 2> {
+    typedef T type __attribute__((ext_vector_type(2)));
+};
+
 // Block solve with NR right-hand sides sharing each step record; x segments
 // interleaved in LDS as xs[i*NR + r] (i < k+2, slot k holds 0).  A launch of
 // nr_slots right-hand sides runs nr_slots / NR waves per partition
@@ -424,45 +437,56 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // accumulator), then the dot products.  Empty row slots name slot
         // k+1, so no lane tests for them.
         //
-        // One RHS (the latency-bound case): lane 0 of the group starts its
-        // dot product from the row's accumulator, the other lanes from the
-        // zero slot k; lane 0 writes the row and every other lane its own
-        // spare slot k+2+lane, so neither needs a branch.  With several
-        // lanes per row both addresses select between the row's LDS address
-        // and an address fixed for the lane (opaque to the compiler, which
-        // would otherwise form each address after its select), and the
-        // addresses of step d+1 are formed during step d, after its LDS
-        // reads are issued, off the path from step d's write to step d+1's
-        // reads; the empty asm statement keeps them there (the scheduler
-        // otherwise hoisted them to the start of a ring turn, where they
-        // waited for ring loads issued only a few steps before).  Measured on
-        // a 1M-row banded matrix at 4 lanes per row: 44.1 / 47.6 us per pass
-        // against 44.7 / 48.8.
-        using LdsT = __attribute__((address_space(3))) T;
-        LdsT *const xl = (LdsT *)xs;
-        LdsT *zero_p = xl + k, *spare_p = xl + (k + 2 + lane);
+        // One or two RHS with several lanes per row (the latency-bound
+        // cases): lane 0 of the group starts its dot product from the row's
+        // accumulator, the other lanes from the zero slot k, so no lane
+        // branches.  With one RHS lane 0 writes the row and every other lane
+        // its own spare slot k+2+lane; with two every lane of the slot writes
+        // the same sums to the row.  The addresses select between the row's
+        // LDS address and an address fixed for the lane (opaque to the
+        // compiler, which would otherwise form each address after its
+        // select), and the addresses of step d+1 are formed during step d,
+        // after its LDS reads are issued, off the path from step d's write to
+        // step d+1's reads; the empty asm statement keeps them there (the
+        // scheduler otherwise hoisted them, and the loads they wait for, to
+        // the start of a ring turn, where they waited for ring loads issued
+        // only a few steps before).  An LDS slot's NR values move as one
+        // vector.  Measured on a 1M-row banded matrix at 4 lanes per row, one
+        // RHS: 44.1 / 47.6 us per pass against 44.7 / 48.8.
+        using SlotT = typename SlotVec<T, (NR <= 2 ? NR : 1)>::type;
+        using LdsV = __attribute__((address_space(3))) SlotT;
+        LdsV *const xl = (LdsV *)xs;
+        LdsV *zero_p = xl + k, *spare_p = xl + (k + 2 + lane);
         asm volatile("" : "+v"(zero_p), "+v"(spare_p));
-        LdsT *a_col = xl, *a_acc = xl, *a_wr = xl;
+        LdsV *a_col = xl, *a_acc = xl, *a_wr = xl;
         auto addr = [&](int d) __attribute__((always_inline)) {
             unsigned row = rrow[d], col = rc[d][0];
             asm volatile("" : "+v"(row), "+v"(col));
-            LdsT *const rp = xl + row;
+            LdsV *const rp = xl + row;
             a_col = xl + col;
             a_acc = li == 0 ? rp : zero_p;
-            a_wr = li == 0 ? rp : spare_p;
+            a_wr = NR == 1 && li != 0 ? spare_p : rp;
         };
         // (one lane per row keeps forming its addresses within the step:
         // a 2D Laplacian's pass 2 measured 80 us piped against 74)
-        constexpr bool piped = NR == 1 && M == 1 && G > 1;
+        constexpr bool piped = NR <= 2 && M == 1 && G > 1;
         if constexpr (piped) addr(0);
         auto step = [&](int d) __attribute__((always_inline)) {
             if constexpr (piped) {
-                const T xv = *a_col;
-                T t = *a_acc;
-                LdsT *const wr = a_wr;
+                const SlotT xv = *a_col;
+                SlotT t = *a_acc;
+                LdsV *const wr = a_wr;
                 addr((d + 1) % D);
-                t = fma(-rv[d][0], xv, t);
-                t = group_sum<G>(t);
+                if constexpr (NR == 1) {
+                    t = fma(-rv[d][0], xv, t);
+                    t = group_sum<G>(t);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < NR; r++) {
+                        t[r] = fma(-rv[d][0], xv[r], t[r]);
+                        t[r] = group_sum<G>(t[r]);
+                    }
+                }
                 *wr = t;
                 return;
             }
