@@ -471,12 +471,18 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // a 2D Laplacian's pass 2 measured 80 us piped against 74)
         constexpr bool piped = NR <= 2 && M == 1 && G > 1;
         if constexpr (piped) addr(0);
-        auto step = [&](int d) __attribute__((always_inline)) {
+        // mid(): ring refills placed after the step's LDS reads (piped
+        // steps only; the scheduling barriers keep them there, in the reads'
+        // latency, rather than between the previous write and these reads)
+        auto step = [&](int d, auto &&mid) __attribute__((always_inline)) {
             if constexpr (piped) {
                 const SlotT xv = *a_col;
                 SlotT t = *a_acc;
                 LdsV *const wr = a_wr;
+                __builtin_amdgcn_sched_barrier(0);
+                mid();
                 addr((d + 1) % D);
+                __builtin_amdgcn_sched_barrier(0);
                 if constexpr (NR == 1) {
                     t = fma(-rv[d][0], xv, t);
                     t = group_sum<G>(t);
@@ -525,17 +531,30 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         auto refill = [&](int d, int sidx) __attribute__((always_inline)) {
             load_step(d, (unsigned)(sidx + D < nst ? sidx + D : nst - 1));
         };
-        int base = early ? 0 : -D;
+        const int base0 = early ? 0 : -D;
+        int base = base0;
         for (; base + D <= nst; base += D) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                step(d);
-                refill(d, base + d);
+                if constexpr (piped) {
+                    // refill the previous step's slot (its last use was that
+                    // step's FMA); slot D-1 of a turn is refilled in the next
+                    // turn's first step, which the tail never needs
+                    step(d, [&] {
+                        if (d > 0)
+                            refill(d - 1, base + d - 1);
+                        else if (base != base0)
+                            refill(D - 1, base - 1);
+                    });
+                } else {
+                    step(d, [] {});
+                    refill(d, base + d);
+                }
             }
         }
 #pragma unroll
         for (int d = 0; d < D; d++)
-            if (base + d < nst) step(d);
+            if (base + d < nst) step(d, [] {});
     }
 epilogue:
     __syncthreads();
