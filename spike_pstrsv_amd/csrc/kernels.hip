@@ -471,9 +471,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // a 2D Laplacian's pass 2 measured 80 us piped against 74)
         constexpr bool piped = NR <= 2 && M == 1 && G > 1;
         if constexpr (piped) addr(0);
-        // mid(): ring refills placed after the step's LDS reads (piped
-        // steps only; the scheduling barriers keep them there, in the reads'
-        // latency, rather than between the previous write and these reads)
+        // mid(): ring refills placed after the step's LDS reads (the
+        // scheduling barriers keep them there, in the reads' latency, rather
+        // than between the previous write and these reads)
         auto step = [&](int d, auto &&mid) __attribute__((always_inline)) {
             if constexpr (piped) {
                 const SlotT xv = *a_col;
@@ -504,6 +504,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 for (int r = 0; r < NR; r++) xv[m][r] = xs[(unsigned)rc[d][m] * NR + r];
             if constexpr (NR == 1) {
                 t[0] = xs[li == 0 ? row : (unsigned)k];
+                __builtin_amdgcn_sched_barrier(0);
+                mid();
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int m = 0; m < M; m++) t[0] = fma(-rv[d][m], xv[m][0], t[0]);
                 t[0] = group_sum<G>(t[0]);
@@ -516,6 +519,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 const unsigned arow = li == 0 ? row : (unsigned)k;
 #pragma unroll
                 for (int r = 0; r < NR; r++) t[r] = xs[arow * NR + r];
+                __builtin_amdgcn_sched_barrier(0);
+                mid();
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
 #pragma unroll
@@ -536,20 +542,15 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         for (; base + D <= nst; base += D) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
-                if constexpr (piped) {
-                    // refill the previous step's slot (its last use was that
-                    // step's FMA); slot D-1 of a turn is refilled in the next
-                    // turn's first step, which the tail never needs
-                    step(d, [&] {
-                        if (d > 0)
-                            refill(d - 1, base + d - 1);
-                        else if (base != base0)
-                            refill(D - 1, base - 1);
-                    });
-                } else {
-                    step(d, [] {});
-                    refill(d, base + d);
-                }
+                // refill the previous step's slot (its last use was that
+                // step's FMA); slot D-1 of a turn is refilled in the next
+                // turn's first step, which the tail never needs
+                step(d, [&] {
+                    if (d > 0)
+                        refill(d - 1, base + d - 1);
+                    else if (base != base0)
+                        refill(D - 1, base - 1);
+                });
             }
         }
 #pragma unroll
