@@ -437,8 +437,8 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // accumulator), then the dot products.  Empty row slots name slot
         // k+1, so no lane tests for them.
         //
-        // One or two RHS with several lanes per row (the latency-bound
-        // cases): lane 0 of the group starts its dot product from the row's
+        // One or two RHS per wave (the latency-bound cases): lane 0 of the
+        // group starts its dot product from the row's
         // accumulator, the other lanes from the zero slot k, so no lane
         // branches.  With one RHS lane 0 writes the row and every other lane
         // its own spare slot k+2+lane; with two every lane of the slot writes
@@ -467,9 +467,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             a_acc = li == 0 ? rp : zero_p;
             a_wr = NR == 1 && li != 0 ? spare_p : rp;
         };
-        // (one lane per row keeps forming its addresses within the step:
-        // a 2D Laplacian's pass 2 measured 80 us piped against 74)
-        constexpr bool piped = NR <= 2 && M == 1 && G > 1;
+        constexpr bool piped = NR <= 2 && M == 1;
         if constexpr (piped) addr(0);
         // mid(): ring refills placed after the step's LDS reads (the
         // scheduling barriers keep them there, in the reads' latency, rather
