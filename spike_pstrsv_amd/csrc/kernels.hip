@@ -330,6 +330,14 @@ template <class T>
 struct SlotVec<T, 2> {
     typedef T type __attribute__((ext_vector_type(2)));
 };
+template <class T>
+struct SlotVec<T, 4> {
+    typedef T type __attribute__((ext_vector_type(4)));
+};
+template <class T>
+struct SlotVec<T, 8> {
+    typedef T type __attribute__((ext_vector_type(8)));
+};
 
 // Block solve with NR right-hand sides sharing each step record; x segments
 // interleaved in LDS as xs[i*NR + r] (i < k+2, slot k holds 0).  A launch of
@@ -437,23 +445,25 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         // accumulator), then the dot products.  Empty row slots name slot
         // k+1, so no lane tests for them.
         //
-        // One or two RHS per wave (the latency-bound cases): lane 0 of the
-        // group starts its dot product from the row's
+        // Lane 0 of the group starts its dot product from the row's
         // accumulator, the other lanes from the zero slot k, so no lane
         // branches.  With one RHS lane 0 writes the row and every other lane
-        // its own spare slot k+2+lane; with two every lane of the slot writes
-        // the same sums to the row.  The addresses select between the row's
-        // LDS address and an address fixed for the lane (opaque to the
-        // compiler, which would otherwise form each address after its
-        // select), and the addresses of step d+1 are formed during step d,
-        // after its LDS reads are issued, off the path from step d's write to
-        // step d+1's reads; the empty asm statement keeps them there (the
-        // scheduler otherwise hoisted them, and the loads they wait for, to
-        // the start of a ring turn, where they waited for ring loads issued
-        // only a few steps before).  An LDS slot's NR values move as one
-        // vector.  Measured on a 1M-row banded matrix at 4 lanes per row, one
-        // RHS: 44.1 / 47.6 us per pass against 44.7 / 48.8.
-        using SlotT = typename SlotVec<T, (NR <= 2 ? NR : 1)>::type;
+        // its own spare slot k+2+lane; with several every lane of the slot
+        // writes the same sums to the row.  Each RHS gets the same
+        // arithmetic, so a column gives the same bits in any tile.  The NR
+        // values of an LDS slot move as one vector.  The addresses select
+        // between the row's LDS address and an address fixed for the lane
+        // (opaque to the compiler, which would otherwise form each address
+        // after its select), and the addresses of step d+1 are formed during
+        // step d, after its LDS reads are issued, off the path from step d's
+        // write to step d+1's reads; the empty asm statement keeps them there
+        // (the scheduler otherwise hoisted them, and the ring waits they
+        // carry, to the start of a ring turn, where they waited for ring
+        // loads issued only a few steps before).  Measured on a 1M-row banded
+        // matrix at 4 lanes per row, one RHS: 44.1 / 47.6 us per pass against
+        // 44.7 / 48.8.
+        static_assert(M == 1, "one step entry per lane");
+        using SlotT = typename SlotVec<T, NR>::type;
         using LdsV = __attribute__((address_space(3))) SlotT;
         LdsV *const xl = (LdsV *)xs;
         LdsV *zero_p = xl + k, *spare_p = xl + (k + 2 + lane);
@@ -467,68 +477,29 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
             a_acc = li == 0 ? rp : zero_p;
             a_wr = NR == 1 && li != 0 ? spare_p : rp;
         };
-        constexpr bool piped = NR <= 2 && M == 1;
-        if constexpr (piped) addr(0);
+        addr(0);
         // mid(): ring refills placed after the step's LDS reads (the
         // scheduling barriers keep them there, in the reads' latency, rather
         // than between the previous write and these reads)
         auto step = [&](int d, auto &&mid) __attribute__((always_inline)) {
-            if constexpr (piped) {
-                const SlotT xv = *a_col;
-                SlotT t = *a_acc;
-                LdsV *const wr = a_wr;
-                __builtin_amdgcn_sched_barrier(0);
-                mid();
-                addr((d + 1) % D);
-                __builtin_amdgcn_sched_barrier(0);
-                if constexpr (NR == 1) {
-                    t = fma(-rv[d][0], xv, t);
-                    t = group_sum<G>(t);
-                } else {
-#pragma unroll
-                    for (int r = 0; r < NR; r++) {
-                        t[r] = fma(-rv[d][0], xv[r], t[r]);
-                        t[r] = group_sum<G>(t[r]);
-                    }
-                }
-                *wr = t;
-                return;
-            }
-            const unsigned row = rrow[d];
-            T xv[M][NR], t[NR];
-#pragma unroll
-            for (int m = 0; m < M; m++)
-#pragma unroll
-                for (int r = 0; r < NR; r++) xv[m][r] = xs[(unsigned)rc[d][m] * NR + r];
+            const SlotT xv = *a_col;
+            SlotT t = *a_acc;
+            LdsV *const wr = a_wr;
+            __builtin_amdgcn_sched_barrier(0);
+            mid();
+            addr((d + 1) % D);
+            __builtin_amdgcn_sched_barrier(0);
             if constexpr (NR == 1) {
-                t[0] = xs[li == 0 ? row : (unsigned)k];
-                __builtin_amdgcn_sched_barrier(0);
-                mid();
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int m = 0; m < M; m++) t[0] = fma(-rv[d][m], xv[m][0], t[0]);
-                t[0] = group_sum<G>(t[0]);
-                xs[li == 0 ? row : (unsigned)(k + 2 + lane)] = t[0];
+                t = fma(-rv[d][0], xv, t);
+                t = group_sum<G>(t);
             } else {
-                // Several RHS: the same arithmetic per RHS as one RHS above
-                // (so a column gives the same bits in any tile); every lane
-                // of a row slot holds the same sums and writes them to the
-                // row (slot k+1 for an empty slot): no branch on the chain.
-                const unsigned arow = li == 0 ? row : (unsigned)k;
-#pragma unroll
-                for (int r = 0; r < NR; r++) t[r] = xs[arow * NR + r];
-                __builtin_amdgcn_sched_barrier(0);
-                mid();
-                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-#pragma unroll
-                    for (int m = 0; m < M; m++) t[r] = fma(-rv[d][m], xv[m][r], t[r]);
+                    t[r] = fma(-rv[d][0], xv[r], t[r]);
                     t[r] = group_sum<G>(t[r]);
                 }
-#pragma unroll
-                for (int r = 0; r < NR; r++) xs[row * NR + r] = t[r];
             }
+            *wr = t;
         };
         // Refill slot d with step sidx + D (clamped to the last step), after
         // the last use of the slot's registers.
