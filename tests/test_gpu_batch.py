@@ -132,3 +132,24 @@ def test_batch_rejects_bad_input(native):
     ja[5] = 500                                          # column out of range
     with pytest.raises(native.SpikeError):
         native.Batch("L", [(m.a, m.ia, ja, 1)])
+
+
+def test_tiles_give_the_same_bits(native):
+    """One matrix in groups of 1, 2, 4 and 8 right-hand sides (block-solve
+    tiles of that many RHS per wave): a column shared by all of them comes
+    out bit for bit the same, as the step arithmetic is per RHS."""
+    m = gen.banded(3000, 40, 10, 0.02, gen.SEED_BASE + 260)
+    counts = (1, 2, 4, 8)
+    bt = native.Batch("L", [(m.a, m.ia, m.ja, r) for r in counts], rows_per_partition=512)
+    col = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 261).ravel()
+    b = np.zeros(bt.offsets[-1])
+    for i, r in enumerate(counts):
+        B = gen.rhs(m.n, r, seed=gen.SEED_BASE + 262 + i).reshape(r, m.n)
+        B[r - 1] = col
+        b[bt.offsets[i]:bt.offsets[i + 1]] = B.ravel()
+    x = bt.solve(b)
+    ref = x[bt.offsets[0]:bt.offsets[1]]
+    assert relerr(ref, orc.ldsol(m.a, m.ia, m.ja, col)) <= TOL
+    for i, r in enumerate(counts[1:], 1):
+        X = x[bt.offsets[i]:bt.offsets[i + 1]].reshape(r, m.n)
+        assert np.array_equal(X[r - 1], ref), r
