@@ -304,8 +304,9 @@ def main() -> int:
         print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (the Spike path would shard by partition ranges "
-              f"across GPUs, see DESIGN.md); --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (the Spike path would shard by contiguous partition "
+              f"ranges across GPUs, exchanging only the reduced right-hand side); --gpus {args.gpus} is not "
+              f"supported", file=sys.stderr)
         return 2
 
     import numpy as np
