@@ -304,7 +304,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes": src.numel() * 8, "copies": 3}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (the Spike path would shard by contiguous partition "
+        print(f"bench.py: runs on one GPU only (the Spike path would shard by contiguous partition "
               f"ranges across GPUs, exchanging only the reduced right-hand side); --gpus {args.gpus} is not "
               f"supported", file=sys.stderr)
         return 2
