@@ -372,19 +372,14 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     const __amdgpu_buffer_rsrc_t bval = make_rsrc(VT<T>::val(A) + (size_t)sb * M * 64, nstu * M * 64u * TB);
     const __amdgpu_buffer_rsrc_t bcol = make_rsrc(A.st.col + (size_t)sb * M * 64, nstu * M * 128u);
     const __amdgpu_buffer_rsrc_t brow = make_rsrc(A.st.row + (size_t)sb * R, nstu * R * 2u);
-    // Register ring of D step records (slot d holds step base + d).  With
-    // one RHS per wave and several lanes per row the first D records are
-    // loaded before the right-hand sides, so their latency overlaps with that
-    // prologue (measured 4 us per solve faster on a 1M-row banded matrix at
-    // 4 lanes per row).  Otherwise (more RHS per wave, or one lane per row)
-    // that raised the register count and slowed the step loop (2 RHS per wave
-    // on the banded matrix: 77 us per pass against 58; a 2D Laplacian at one
-    // lane per row: 87 against 78), so there the ring starts as empty padding
-    // steps (column k, row k+1) and the loop starts D steps early, its first
-    // turn only issuing loads.  The loop runs whole
-    // ring turns (the last < D steps run in the unrolled tail below), each
-    // step refilling its slot with the record D steps ahead (clamped to the
-    // last step), so loads stay D steps in flight.
+    // Register ring of D step records (slot d holds step base + d).  The
+    // first D records are loaded before the right-hand sides, so their
+    // latency overlaps with that prologue (measured on a 1M-row banded
+    // matrix: 4 us per solve at one RHS; 38.6 -> 34.8 ms for a batch of 4096
+    // matrices in tiles of up to 8 RHS).  The loop runs whole ring turns (the
+    // last < D steps run in the unrolled tail below), each step refilling a
+    // slot with the record D steps ahead (clamped to the last step), so
+    // loads stay D steps in flight.
     // Columns and rows are held 32-bit: the 16-bit buffer loads zero-extend,
     // so no instruction widens them at their use (16-bit ring fields cost two
     // masking instructions per step on the step chain).
@@ -399,22 +394,9 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         }
         rrow[d] = buf_u16(brow, (unsigned)slot * 2u, st * (unsigned)(R * 2));
     };
-    constexpr bool early = NR == 1 && G > 1;
-    if constexpr (early) {
-        if (nst > 0)
+    if (nst > 0)
 #pragma unroll
-            for (int d = 0; d < D; d++) load_step(d, (unsigned)(d < nst ? d : nst - 1));
-    } else {
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                rv[d][m] = (T)0;
-                rc[d][m] = (unsigned)k;
-            }
-            rrow[d] = (unsigned)(k + 1);
-        }
-    }
+        for (int d = 0; d < D; d++) load_step(d, (unsigned)(d < nst ? d : nst - 1));
 
     // Pass 1 with one RHS: the local rows of the first 64*UP dependents
     // (written back in the epilogue) are loaded here, with the right-hand
@@ -506,8 +488,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
         auto refill = [&](int d, int sidx) __attribute__((always_inline)) {
             load_step(d, (unsigned)(sidx + D < nst ? sidx + D : nst - 1));
         };
-        const int base0 = early ? 0 : -D;
-        int base = base0;
+        int base = 0;
         for (; base + D <= nst; base += D) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
@@ -517,7 +498,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
                 step(d, [&] {
                     if (d > 0)
                         refill(d - 1, base + d - 1);
-                    else if (base != base0)
+                    else if (base > 0)
                         refill(D - 1, base - 1);
                 });
             }
