@@ -1008,6 +1008,13 @@ __global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int4 *__re
     const bool ok = rc.x >= 0;   // padding records have row -1 and no entries
     const int row = rc.x;
     const int gm = flip ? n - 1 - row : row;
+    // 1/diag and b of the row (row 0 for padding, unused), loaded with the
+    // entries rather than after the gathers and the group sum
+    const int rs = ok ? row : 0, gs = flip ? n - 1 - rs : rs;
+    const T d = invd[rs];
+    double bv[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) bv[t] = b[(size_t)min(t, nr - 1) * ldb + gs];
     T acc[NR];
 #pragma unroll
     for (int t = 0; t < NR; t++) acc[t] = (T)0;
@@ -1026,11 +1033,13 @@ __global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int4 *__re
             c[u] = flip ? n - 1 - cc : cc;
             w[u] = in ? wl : (T)0;
         }
+        // (columns t >= nr read column nr-1: unconditional, so the gathers
+        // are issued together; their sums are not stored)
         T xv[U][NR];
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int t = 0; t < NR; t++) xv[u][t] = t < nr ? (T)x[(size_t)t * ldx + c[u]] : (T)0;
+            for (int t = 0; t < NR; t++) xv[u][t] = (T)x[(size_t)min(t, nr - 1) * ldx + c[u]];
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
@@ -1040,10 +1049,9 @@ __global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int4 *__re
 #pragma unroll
     for (int t = 0; t < NR; t++) acc[t] = group_sum<G>(acc[t]);
     if (ok && li == 0) {
-        const T d = invd[row];
 #pragma unroll
         for (int t = 0; t < NR; t++)
-            if (t < nr) x[(size_t)t * ldx + gm] = (double)(((T)b[(size_t)t * ldb + gm] - acc[t]) * d);
+            if (t < nr) x[(size_t)t * ldx + gm] = (double)(((T)bv[t] - acc[t]) * d);
     }
 }
 
