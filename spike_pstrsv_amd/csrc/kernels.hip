@@ -890,11 +890,17 @@ __global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ pa
     const DevPart pt = parts[blockIdx.x];
     const RhsMap m = rhs_map(A, pt);
     // U entries per trip at clamped indices: values and columns, then the
-    // gathers, then the sums in entry order
-    constexpr int U = NR >= 8 ? 2 : 8 / NR;
+    // gathers, then the sums in entry order (8 per trip, 4 with 8 RHS to
+    // keep four waves per SIMD: a coupling row of the banded matrices has up
+    // to about 16 entries, and each trip is two dependent round trips)
+    constexpr int U = NR >= 8 ? 4 : 8;
     for (int c = pt.cbeg + threadIdx.x; c < pt.cbeg + pt.ccnt; c += blockDim.x) {
         const int g = rows[c];
         const int j0 = ptr[c], j1 = ptr[c + 1];
+        // the row's right-hand sides, loaded with its first entries
+        double bv[NR];
+#pragma unroll
+        for (int t = 0; t < NR; t++) bv[t] = m.bv(t, g);
         double acc[NR];
 #pragma unroll
         for (int t = 0; t < NR; t++) acc[t] = 0.0;
@@ -918,7 +924,7 @@ __global__ __launch_bounds__(256) void k_coupling(const DevPart *__restrict__ pa
                     for (int t = 0; t < NR; t++) acc[t] += w[u] * xv[u][t];
         }
 #pragma unroll
-        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = m.bv(t, g) - acc[t];
+        for (int t = 0; t < NR; t++) bc[(size_t)c * NR + t] = bv[t] - acc[t];
     }
 }
 
