@@ -951,6 +951,12 @@ __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ pa
     for (int base = 0; base < pt.k; base += 256 / G) {
         const int lr = base + threadIdx.x / G;
         const int row = pt.s + lr;
+        // the row's right-hand sides, loaded with its entries rather than
+        // after the gathers (columns beyond nr read column nr-1, unused)
+        const int gb = m.at(pt.s + min(lr, pt.k - 1));
+        double bv[NR];
+#pragma unroll
+        for (int t = 0; t < NR; t++) bv[t] = m.b[(size_t)min(c0 + t, m.nr - 1) * m.ldb + gb];
         double acc[NR];
 #pragma unroll
         for (int t = 0; t < NR; t++) acc[t] = 0.0;
@@ -969,7 +975,7 @@ __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ pa
                 for (int u = 0; u < U; u++)
 #pragma unroll
                     for (int t = 0; t < NR; t++)
-                        xv[u][t] = c0 + t < m.nr ? m.x[(size_t)(c0 + t) * m.ldx + cm[u]] : 0.0;
+                        xv[u][t] = m.x[(size_t)min(c0 + t, m.nr - 1) * m.ldx + cm[u]];
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (e + u * G < e1)
@@ -983,8 +989,7 @@ __global__ __launch_bounds__(256) void k_residual(const DevPart *__restrict__ pa
             const int gm = m.at(row);
 #pragma unroll
             for (int t = 0; t < NR; t++)
-                if (c0 + t < m.nr)
-                    rp[(size_t)(c0 + t) * ld_r + gm] = m.b[(size_t)(c0 + t) * m.ldb + gm] - acc[t];
+                if (c0 + t < m.nr) rp[(size_t)(c0 + t) * ld_r + gm] = bv[t] - acc[t];
         }
     }
 }
