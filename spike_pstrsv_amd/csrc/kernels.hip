@@ -1235,14 +1235,17 @@ __global__ __launch_bounds__(256) void k_interleave(int n, int nr, int nrp, int 
     const int gm = flip ? n - 1 - i : i;
     dst[t] = r < nr ? src[(size_t)r * ld + gm] : 0.0;
 }
-// x (caller layout) = v, or x += v with accum.
+// x (caller layout) = v, or x += v with accum.  Consecutive lanes take
+// consecutive right-hand sides of a row, so each wave reads one contiguous
+// stretch of v (a column-by-column order would read every row's nrp-value
+// record once per column).
 __global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int nrp, int flip, const double *__restrict__ v,
                                                       double *x, int ld, int accum)
 {
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= (size_t)n * nr) return;
-    const int r = (int)(t / n), gm = (int)(t % n);
-    const int i = flip ? n - 1 - gm : gm;
+    const int i = (int)(t / nr), r = (int)(t % nr);
+    const int gm = flip ? n - 1 - i : i;
     const double w = v[(size_t)i * nrp + r];
     double *dst = x + (size_t)r * ld + gm;
     *dst = accum ? *dst + w : w;
@@ -1254,6 +1257,26 @@ __global__ __launch_bounds__(256) void k_axpy(int n, int nr, const double *__res
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     for (int t = 0; t < nr; t++) x[(size_t)t * ldx + i] += d[(size_t)t * ldd + i];
+}
+
+// One column per grid row, 1024 entries per workgroup: all four loads of a
+// lane are issued before its stores.
+__global__ __launch_bounds__(256) void k_copy_cols(int n, ColCopy cc)
+{
+    const double *__restrict__ src = cc.src[blockIdx.y];
+    double *__restrict__ dst = cc.dst[blockIdx.y];
+    const int i0 = blockIdx.x * 1024 + threadIdx.x;
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = i0 + u * 256;
+        v[u] = i < n ? src[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = i0 + u * 256;
+        if (i < n) dst[i] = v[u];
+    }
 }
 
 // ---------------------------------------------------------------- spikes
@@ -1587,6 +1610,14 @@ hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int l
 {
     if (n <= 0 || nr <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, s, n, nr, d, ldd, x, ldx);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_cols(int n, int ncols, const ColCopy &cc, hipStream_t s)
+{
+    if (n <= 0 || ncols <= 0) return hipSuccess;
+    if (ncols > kMaxCopyCols) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)((n + 1023) / 1024), (unsigned)ncols), dim3(256), 0, s, n, cc);
     return hipGetLastError();
 }
 

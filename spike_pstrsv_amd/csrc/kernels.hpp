@@ -175,6 +175,15 @@ hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v
 // x[:, t] += d[:, t], t < nr.
 hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int ldx, hipStream_t s);
 
+// Copies ncols (<= kMaxCopyCols) columns of n doubles, dst[c][i] = src[c][i],
+// in one launch (the columns may come from different allocations).
+constexpr int kMaxCopyCols = 32;
+struct ColCopy {
+    const double *src[kMaxCopyCols];
+    double *dst[kMaxCopyCols];
+};
+hipError_t launch_copy_cols(int n, int ncols, const ColCopy &cc, hipStream_t s);
+
 // Largest dynamic LDS a block-solve workgroup may use, in bytes.
 size_t block_solve_lds_limit();
 

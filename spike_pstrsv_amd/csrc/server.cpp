@@ -15,6 +15,7 @@
 // batching).  Completion is observed by polling events; latency = completion
 // time - submit time.
 #include "spike_pstrsv.h"
+#include "kernels.hpp"
 
 #include <hip/hip_runtime_api.h>
 
@@ -135,13 +136,24 @@ int spike_server_s::launch(const Job &j)
     const int n = mat_n[j.id];
     hipStream_t s = streams[j.id % streams.size()];
     double *mb = stage[j.id], *mx = stage[j.id] + (size_t)n * opts.max_batch_rhs;
-    int c = 0;
-    for (const Request &r : j.reqs) {
-        if (hipMemcpy2DAsync(mb + (size_t)c * n, sizeof(double) * n, r.b, sizeof(double) * r.ldb, sizeof(double) * n,
-                             r.nrhs, hipMemcpyDefault, s) != hipSuccess)
-            return SPIKE_ERR_DEVICE;
-        c += r.nrhs;
-    }
+    // every column of the merged requests moves in one copy launch (per
+    // spike::kMaxCopyCols columns) each way
+    auto copy_cols = [&](bool in) {
+        spike::ColCopy cc{};
+        int c = 0, k = 0;
+        for (const Request &r : j.reqs)
+            for (int t = 0; t < r.nrhs; t++, c++) {
+                double *st = (in ? mb : mx) + (size_t)c * n;
+                cc.src[k] = in ? r.b + (size_t)t * r.ldb : st;
+                cc.dst[k] = in ? st : r.x + (size_t)t * r.ldx;
+                if (++k == spike::kMaxCopyCols) {
+                    if (spike::launch_copy_cols(n, k, cc, s) != hipSuccess) return false;
+                    k = 0;
+                }
+            }
+        return spike::launch_copy_cols(n, k, cc, s) == hipSuccess;
+    };
+    if (!copy_cols(true)) return SPIKE_ERR_DEVICE;
     // solve a power-of-two column count (the kernels' slot count): columns
     // past j.cols hold earlier right-hand sides and their results are unused
     int padded = 1;
@@ -149,13 +161,7 @@ int spike_server_s::launch(const Job &j)
     padded = std::min(padded, opts.max_batch_rhs);
     int rc = spike_factor_solve(mats[j.id], padded, mb, n, mx, n, s);
     if (rc != SPIKE_OK) return rc;
-    c = 0;
-    for (const Request &r : j.reqs) {
-        if (hipMemcpy2DAsync(r.x, sizeof(double) * r.ldx, mx + (size_t)c * n, sizeof(double) * n, sizeof(double) * n,
-                             r.nrhs, hipMemcpyDefault, s) != hipSuccess)
-            return SPIKE_ERR_DEVICE;
-        c += r.nrhs;
-    }
+    if (!copy_cols(false)) return SPIKE_ERR_DEVICE;
     return hipEventRecord(j.ev, s) == hipSuccess ? SPIKE_OK : SPIKE_ERR_DEVICE;
 }
 
