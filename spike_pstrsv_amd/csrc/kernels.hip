@@ -1225,30 +1225,66 @@ __global__ __launch_bounds__(256) void k_not_ready(size_t len, double *v, int *a
 
 // Interleave / de-interleave between the caller's column-major layout
 // (caller index flip ? n-1-i : i, leading dimension ld, nr columns) and
-// v[i*NRP + r] over lower-form rows i.  Columns r >= nr are filled with 0.
-__global__ __launch_bounds__(256) void k_interleave(int n, int nr, int nrp, int flip, const double *__restrict__ src,
+// v[i*nrp + r] over lower-form rows i (nrp = 1 << lgp <= 32).  A workgroup
+// moves one tile of 2048 values, TR = 2048 / nrp rows by nrp columns, through
+// LDS, so both the column side (TR contiguous rows per column) and the
+// interleaved side (one contiguous stretch) are accessed in whole lines.  The
+// LDS tile is column-major with one pad value per column.
+constexpr int kTile = 2048;
+
+// Columns r >= nr are filled with 0.
+__global__ __launch_bounds__(256) void k_interleave(int n, int nr, int lgp, int flip, const double *__restrict__ src,
                                                     int ld, double *__restrict__ dst)
 {
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (size_t)n * nrp) return;
-    const int i = (int)(t / nrp), r = (int)(t % nrp);
-    const int gm = flip ? n - 1 - i : i;
-    dst[t] = r < nr ? src[(size_t)r * ld + gm] : 0.0;
+    __shared__ double tile[kTile + 32];
+    const int nrp = 1 << lgp, lgr = 11 - lgp, TR = 1 << lgr;
+    const int i0 = blockIdx.x << lgr;
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int f = threadIdx.x + 256 * u, r = f >> lgr, i = i0 + (f & (TR - 1));
+        v[u] = r < nr && i < n ? src[(size_t)r * ld + (flip ? n - 1 - i : i)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int f = threadIdx.x + 256 * u;
+        tile[(f >> lgr) * (TR + 1) + (f & (TR - 1))] = v[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int g = threadIdx.x + 256 * u, row = g >> lgp;
+        if (i0 + row < n) dst[((size_t)i0 << lgp) + g] = tile[(g & (nrp - 1)) * (TR + 1) + row];
+    }
 }
-// x (caller layout) = v, or x += v with accum.  Consecutive lanes take
-// consecutive right-hand sides of a row, so each wave reads one contiguous
-// stretch of v (a column-by-column order would read every row's nrp-value
-// record once per column).
-__global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int nrp, int flip, const double *__restrict__ v,
+// x (caller layout) = v, or x += v with accum, for the first nr columns.
+__global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int lgp, int flip, const double *__restrict__ v,
                                                       double *x, int ld, int accum)
 {
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (size_t)n * nr) return;
-    const int i = (int)(t / nr), r = (int)(t % nr);
-    const int gm = flip ? n - 1 - i : i;
-    const double w = v[(size_t)i * nrp + r];
-    double *dst = x + (size_t)r * ld + gm;
-    *dst = accum ? *dst + w : w;
+    __shared__ double tile[kTile + 32];
+    const int nrp = 1 << lgp, lgr = 11 - lgp, TR = 1 << lgr;
+    const int i0 = blockIdx.x << lgr;
+    double w[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int g = threadIdx.x + 256 * u;
+        w[u] = i0 + (g >> lgp) < n ? v[((size_t)i0 << lgp) + g] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int g = threadIdx.x + 256 * u;
+        tile[(g & (nrp - 1)) * (TR + 1) + (g >> lgp)] = w[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int f = threadIdx.x + 256 * u, r = f >> lgr, row = f & (TR - 1), i = i0 + row;
+        if (r < nr && i < n) {
+            double *d = x + (size_t)r * ld + (flip ? n - 1 - i : i);
+            const double t = tile[r * (TR + 1) + row];
+            *d = accum ? *d + t : t;
+        }
+    }
 }
 
 // x[:, t] += d[:, t] for t < nr (refinement update of the level method).
@@ -1587,11 +1623,23 @@ hipError_t launch_sync_free(int npos, int n, int nrp, const LevelArgs &a, const 
     }
 }
 
+namespace {
+// log2 of an interleave width (a power of two <= 32), or -1.
+int il_log2(int nrp)
+{
+    for (int l = 0; l <= 5; l++)
+        if (nrp == 1 << l) return l;
+    return -1;
+}
+}  // namespace
+
 hipError_t launch_interleave(int n, int nr, int nrp, int flip, const double *src, int ld, double *dst, hipStream_t s)
 {
-    const size_t tot = (size_t)n * nrp;
-    if (tot == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_interleave, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, nrp, flip, src, ld,
+    const int lgp = il_log2(nrp);
+    if (lgp < 0 || nr > nrp) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
+    const int tr = kTile >> lgp;
+    hipLaunchKernelGGL(k_interleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, src, ld,
                        dst);
     return hipGetLastError();
 }
@@ -1599,9 +1647,11 @@ hipError_t launch_interleave(int n, int nr, int nrp, int flip, const double *src
 hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v, double *x, int ld, int accum,
                                hipStream_t s)
 {
-    const size_t tot = (size_t)n * nr;
-    if (tot == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nr, nrp, flip, v, x,
+    const int lgp = il_log2(nrp);
+    if (lgp < 0 || nr > nrp) return hipErrorInvalidValue;
+    if (n <= 0 || nr <= 0) return hipSuccess;
+    const int tr = kTile >> lgp;
+    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v, x,
                        ld, accum);
     return hipGetLastError();
 }
