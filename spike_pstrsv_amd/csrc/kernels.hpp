@@ -164,7 +164,7 @@ hipError_t launch_level_il(int r0, int cnt, int nrp, const LevelArgs &a, const d
 // published.  A row whose wait gives up (after a.spin_limit polls) writes
 // NaN and sets *abort_word.  Measured on gfx950 (lower 3D Laplacian, 128^3):
 // 1.18 ms at 8 RHS and 3.14 ms at 32 RHS against 1.72 and 4.49 ms for one
-// launch per level; with one RHS one launch per level stays faster (1.10 ms
+// launch per level (whole solves, with the layout changes of that time); with one RHS one launch per level stays faster (1.10 ms
 // against 1.59), so the single-RHS path keeps it.
 hipError_t launch_sync_free(int npos, int n, int nrp, const LevelArgs &a, const double *bi, double *xi,
                             int *abort_word, hipStream_t s);
