@@ -6,8 +6,9 @@
 // dispatcher thread closes a batching window window_us after the oldest
 // waiting request arrived, merges the window's requests of each matrix into
 // solves of up to max_batch_rhs right-hand sides, and launches them on a pool
-// of streams (matrix id modulo the pool size, so the solves of one factor
-// stay ordered and never share its workspaces).  Right-hand sides are
+// of streams (a matrix with solves in flight keeps their stream, so the
+// solves of one factor stay ordered and never share its workspaces; an idle
+// matrix takes the least busy stream).  Right-hand sides are
 // gathered into the matrix's fixed staging buffers and solutions scattered
 // back, so every solve of a matrix has the same pointers and replays the
 // factor's captured hipGraph.  At most `slots` solves are in flight; requests
@@ -49,10 +50,11 @@ struct InFlight {
     hipEvent_t done;
     std::vector<Request> reqs;
     int64_t rows_rhs;
+    int id, sidx;
 };
 
 struct Job {
-    int id, cols;
+    int id, cols, sidx;
     std::vector<Request> reqs;
     hipEvent_t ev;
 };
@@ -72,6 +74,10 @@ struct spike_server_s {
     std::deque<Request> pending;
     std::vector<InFlight> inflight;
     std::vector<hipStream_t> streams;
+    // Stream choice: a matrix with solves in flight (or being launched) keeps
+    // their stream, so its solves stay ordered; otherwise it takes the stream
+    // with the fewest solves in flight.
+    std::vector<int> mat_sidx, mat_busy, stream_busy;
     std::vector<hipEvent_t> spare_events;
     uint64_t next_ticket = 1;
     std::map<uint64_t, int> finished;     // ticket -> status (until collected)
@@ -91,6 +97,8 @@ struct spike_server_s {
     void dispatch_window(std::unique_lock<std::mutex> &lk);
     void poll_done(std::unique_lock<std::mutex> &lk);
     int launch(const Job &j);
+    int pick_stream(int id);
+    void release_stream(int id, int sidx);
 };
 
 namespace {
@@ -122,6 +130,7 @@ void spike_server_s::poll_done(std::unique_lock<std::mutex> &lk)
             lat.push_back(secs(r.t_submit, now));
         }
         rows_rhs_done += inflight[i].rows_rhs;
+        release_stream(inflight[i].id, inflight[i].sidx);
         t_last = now;
         spare_events.push_back(inflight[i].done);
         inflight[i] = std::move(inflight.back());
@@ -130,11 +139,32 @@ void spike_server_s::poll_done(std::unique_lock<std::mutex> &lk)
     }
 }
 
+// Stream of a new solve of matrix id (mu held); counted until released.
+int spike_server_s::pick_stream(int id)
+{
+    int sidx = mat_busy[id] > 0 ? mat_sidx[id] : -1;
+    if (sidx < 0) {
+        sidx = 0;
+        for (int k = 1; k < (int)streams.size(); k++)
+            if (stream_busy[k] < stream_busy[sidx]) sidx = k;
+    }
+    mat_sidx[id] = sidx;
+    mat_busy[id]++;
+    stream_busy[sidx]++;
+    return sidx;
+}
+
+void spike_server_s::release_stream(int id, int sidx)
+{
+    mat_busy[id]--;
+    stream_busy[sidx]--;
+}
+
 // Launches one merged solve (mu not held): gather, solve, scatter, event.
 int spike_server_s::launch(const Job &j)
 {
     const int n = mat_n[j.id];
-    hipStream_t s = streams[j.id % streams.size()];
+    hipStream_t s = streams[j.sidx];
     double *mb = stage[j.id], *mx = stage[j.id] + (size_t)n * opts.max_batch_rhs;
     // every column of the merged requests moves in one copy launch (per
     // spike::kMaxCopyCols columns) each way
@@ -219,6 +249,7 @@ void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
                 keep.insert(keep.end(), rs.begin() + i, rs.end());
                 break;
             }
+            j.sidx = pick_stream(j.id);
             jobs.push_back(std::move(j));
             free_slots--;
         }
@@ -241,7 +272,8 @@ void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
         if (rcs[k] != SPIKE_OK) {
             error = rcs[k];
             error_msg = spike_last_error();
-            (void)hipStreamSynchronize(streams[j.id % streams.size()]);
+            (void)hipStreamSynchronize(streams[j.sidx]);
+            release_stream(j.id, j.sidx);
             for (const Request &r : j.reqs) finished[r.ticket] = rcs[k];
             spare_events.push_back(j.ev);
             cv_done.notify_all();
@@ -251,6 +283,8 @@ void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
         fl.done = j.ev;
         fl.reqs = std::move(j.reqs);
         fl.rows_rhs = (int64_t)mat_n[j.id] * j.cols;
+        fl.id = j.id;
+        fl.sidx = j.sidx;
         inflight.push_back(std::move(fl));
         solves++;
         if (j.cols > 1) merged_rhs += j.cols;
@@ -335,6 +369,7 @@ int spike_server_create(spike_server_t *out, const spike_server_options_t *opts)
         }
         s->streams.push_back(st);
     }
+    s->stream_busy.assign(s->streams.size(), 0);
     s->worker = std::thread([s]() { s->run(); });
     *out = s;
     return SPIKE_OK;
@@ -393,6 +428,8 @@ int spike_server_add(spike_server_t s, char uplo, int n, const double *a, const 
     s->resident += need;
     s->mats.push_back(f);
     s->mat_n.push_back(n);
+    s->mat_sidx.push_back(-1);
+    s->mat_busy.push_back(0);
     s->mat_bytes.push_back(need);
     s->stage.push_back(stg);
     *id = (int)s->mats.size() - 1;
