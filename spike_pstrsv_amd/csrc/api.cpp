@@ -1243,6 +1243,33 @@ int ensure_level_ws(spike_factor_s *f, int nrhs)
     return SPIKE_OK;
 }
 
+// Level arguments of the interleaved (multi-RHS) launches.
+spike::LevelArgs level_il_args(const spike_factor_s *f)
+{
+    spike::LevelArgs la{};
+    la.rec = f->d_lrec;
+    la.lja = f->d_lja;
+    la.lval = f->d_lval;
+    la.invd = f->d_inv_diag;
+    la.lval32 = f->d_lval32;
+    la.invd32 = f->d_invd32;
+    la.fp32 = f->fp32 ? 1 : 0;
+    la.spin_limit = f->spin_limit;
+    return la;
+}
+
+// xi = M^-1 bi on interleaved vectors of nrp values per row.
+hipError_t level_il_solve(spike_factor_s *f, const spike::LevelArgs &la, int nrp, const double *bi, double *xi,
+                          hipStream_t s)
+{
+    if (f->sync_free) return spike::launch_sync_free(f->lev_ptr.back(), f->plan.n, nrp, la, bi, xi, f->d_abort, s);
+    for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++) {
+        hipError_t e = spike::launch_level_il(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], nrp, la, bi, xi, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
                  hipEvent_t *ev, double *ph)
 {
@@ -1252,24 +1279,11 @@ int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx
         int nrp = 2;
         while (nrp < nr) nrp <<= 1;
         if ((size_t)n * nrp > f->il_cap) return fail(SPIKE_ERR_STATE, "level workspaces not allocated");
-        spike::LevelArgs la{};
-        la.rec = f->d_lrec;
-        la.lja = f->d_lja;
-        la.lval = f->d_lval;
-        la.invd = f->d_inv_diag;
-        la.lval32 = f->d_lval32;
-        la.invd32 = f->d_invd32;
-        la.fp32 = f->fp32 ? 1 : 0;
-        la.spin_limit = f->spin_limit;
+        const spike::LevelArgs la = level_il_args(f);
         const int flip = f->plan.flip ? 1 : 0;
         if (ev) HIP_TRY(hipEventRecord(ev[0], s));
         HIP_TRY(spike::launch_interleave(n, nr, nrp, flip, b, ldb, f->d_bi, s));
-        if (f->sync_free)
-            HIP_TRY(spike::launch_sync_free(f->lev_ptr.back(), n, nrp, la, f->d_bi, f->d_xi, f->d_abort, s));
-        else
-            for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++)
-                HIP_TRY(spike::launch_level_il(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], nrp, la, f->d_bi,
-                                               f->d_xi, s));
+        HIP_TRY(level_il_solve(f, la, nrp, f->d_bi, f->d_xi, s));
         HIP_TRY(spike::launch_deinterleave(n, nr, nrp, flip, f->d_xi, x, ldx, accum, s));
         if (ev) {
             HIP_TRY(hipEventRecord(ev[1], s));
@@ -1376,9 +1390,33 @@ int residual(spike_factor_s *f, const double *b, int ldb, const double *x, int l
 // Solve of nr right-hand sides with the factor's refinement: x_0 = M^-1 b,
 // then refine_steps times r = b - A x (FP64), x += M^-1 r.  The residual's
 // device time is added to ph[2].
+// FP32 level solve of 2..32 right-hand sides with one refinement step, kept
+// interleaved between its launches: x0 = M^-1 b into d_xi, r = b - A x0
+// (FP64) into d_r, d = M^-1 r into d_bi (b is not read after the residual),
+// and x = x0 + d in one de-interleave.
+int solve_levels_refined1(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s)
+{
+    const int n = f->plan.n;
+    int nrp = 2;
+    while (nrp < nr) nrp <<= 1;
+    if ((size_t)n * nrp > f->il_cap) return fail(SPIKE_ERR_STATE, "level workspaces not allocated");
+    const spike::LevelArgs la = level_il_args(f);
+    const int flip = f->plan.flip ? 1 : 0;
+    HIP_TRY(spike::launch_interleave(n, nr, nrp, flip, b, ldb, f->d_bi, s));
+    HIP_TRY(level_il_solve(f, la, nrp, f->d_bi, f->d_xi, s));
+    HIP_TRY(spike::launch_residual_il(n, nrp, f->d_ia, f->d_ja, f->d_a, f->d_bi, f->d_xi, f->d_r, s));
+    HIP_TRY(level_il_solve(f, la, nrp, f->d_r, f->d_bi, s));
+    HIP_TRY(spike::launch_deinterleave_sum(n, nr, nrp, flip, f->d_xi, f->d_bi, x, ldx, s));
+    return SPIKE_OK;
+}
+
 int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s,
                 hipEvent_t *ev, double *ph)
 {
+    // d_r holds n x 32 values on level factors with FP32 values
+    if (f->method == SPIKE_METHOD_LEVELS && f->fp32 && f->refine_steps == 1 && nr >= 2 && nr <= 32 && !ev &&
+        f->mats.empty() && f->d_r)
+        return solve_levels_refined1(f, b, ldb, x, ldx, nr, s);
     int rc = solve_pass(f, b, ldb, x, ldx, nr, 0, s, ev, ph);
     if (rc) return rc;
     const int n = f->plan.n;

@@ -1257,9 +1257,10 @@ __global__ __launch_bounds__(256) void k_interleave(int n, int nr, int lgp, int 
         if (i0 + row < n) dst[((size_t)i0 << lgp) + g] = tile[(g & (nrp - 1)) * (TR + 1) + row];
     }
 }
-// x (caller layout) = v, or x += v with accum, for the first nr columns.
+// x (caller layout) = v, or x += v with accum, for the first nr columns;
+// with v2, v + v2 takes the place of v.
 __global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int lgp, int flip, const double *__restrict__ v,
-                                                      double *x, int ld, int accum)
+                                                      const double *__restrict__ v2, double *x, int ld, int accum)
 {
     __shared__ double tile[kTile + 32];
     const int nrp = 1 << lgp, lgr = 11 - lgp, TR = 1 << lgr;
@@ -1268,7 +1269,10 @@ __global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int lgp, in
 #pragma unroll
     for (int u = 0; u < 8; u++) {
         const int g = threadIdx.x + 256 * u;
-        w[u] = i0 + (g >> lgp) < n ? v[((size_t)i0 << lgp) + g] : 0.0;
+        const size_t idx = ((size_t)i0 << lgp) + g;
+        const bool ok = i0 + (g >> lgp) < n;
+        w[u] = ok ? v[idx] : 0.0;
+        if (v2 != nullptr && ok) w[u] += v2[idx];
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -1285,6 +1289,25 @@ __global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int lgp, in
             *d = accum ? *d + t : t;
         }
     }
+}
+
+// r[i*nrp + t] = b[i*nrp + t] - sum_e a_e x[ja_e*nrp + t] over row i of the
+// lower-form CSR (diagonal included) in FP64, on interleaved vectors: one
+// lane per row and right-hand side, so the x values of one entry are one
+// contiguous segment.  Products are summed in entry order.
+__global__ __launch_bounds__(256) void k_residual_il(int n, int lgp, const int *__restrict__ ia,
+                                                     const int *__restrict__ ja, const double *__restrict__ a,
+                                                     const double *__restrict__ bi, const double *__restrict__ xi,
+                                                     double *__restrict__ r)
+{
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int i = (int)(g >> lgp), t = (int)(g & ((1u << lgp) - 1));
+    if (i >= n) return;
+    const double bv = bi[g];
+    const int e0 = ia[i], e1 = ia[i + 1];
+    double acc = 0.0;
+    for (int e = e0; e < e1; e++) acc = fma(a[e], xi[((size_t)ja[e] << lgp) + t], acc);
+    r[g] = bv - acc;
 }
 
 // x[:, t] += d[:, t] for t < nr (refinement update of the level method).
@@ -1651,8 +1674,32 @@ hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v
     if (lgp < 0 || nr > nrp) return hipErrorInvalidValue;
     if (n <= 0 || nr <= 0) return hipSuccess;
     const int tr = kTile >> lgp;
-    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v, x,
-                       ld, accum);
+    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v,
+                       (const double *)nullptr, x, ld, accum);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave_sum(int n, int nr, int nrp, int flip, const double *v, const double *v2, double *x,
+                                   int ld, hipStream_t s)
+{
+    const int lgp = il_log2(nrp);
+    if (lgp < 0 || nr > nrp || v2 == nullptr) return hipErrorInvalidValue;
+    if (n <= 0 || nr <= 0) return hipSuccess;
+    const int tr = kTile >> lgp;
+    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v, v2,
+                       x, ld, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_il(int n, int nrp, const int *ia, const int *ja, const double *a, const double *bi,
+                              const double *xi, double *r, hipStream_t s)
+{
+    const int lgp = il_log2(nrp);
+    if (lgp < 0) return hipErrorInvalidValue;
+    const size_t tot = (size_t)n << lgp;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_residual_il, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, lgp, ia, ja, a, bi, xi,
+                       r);
     return hipGetLastError();
 }
 

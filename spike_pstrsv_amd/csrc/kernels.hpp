@@ -172,6 +172,14 @@ hipError_t launch_sync_free(int npos, int n, int nrp, const LevelArgs &a, const 
 hipError_t launch_interleave(int n, int nr, int nrp, int flip, const double *src, int ld, double *dst, hipStream_t s);
 hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v, double *x, int ld, int accum,
                                hipStream_t s);
+// x (caller layout) = v + v2 for two interleaved vectors (solution plus
+// refinement correction), first nr columns.
+hipError_t launch_deinterleave_sum(int n, int nr, int nrp, int flip, const double *v, const double *v2, double *x,
+                                   int ld, hipStream_t s);
+// r = b - A x in FP64 on interleaved vectors (nrp values per row, a power of
+// two <= 32) over the lower-form CSR (ia, ja, a; diagonal included).
+hipError_t launch_residual_il(int n, int nrp, const int *ia, const int *ja, const double *a, const double *bi,
+                              const double *xi, double *r, hipStream_t s);
 // x[:, t] += d[:, t], t < nr.
 hipError_t launch_axpy(int n, int nr, const double *d, int ldd, double *x, int ldx, hipStream_t s);
 

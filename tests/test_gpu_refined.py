@@ -125,6 +125,24 @@ def test_levels_chosen_for_lap3d(native, uplo):
         assert relerr(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])) <= TOL
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("nrhs", [3, 32])
+def test_levels_refined_interleaved(native, uplo, nrhs):
+    """FP32 level solve with one refinement step of several right-hand sides:
+    the residual and the correction stay interleaved between launches."""
+    m = gen.lap3d(64)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    B = gen.rhs(m.n, nrhs, seed=gen.SEED_BASE + 160 + nrhs)
+    f = native.Factor(uplo, m.a, m.ia, m.ja, precision="fp32_refined")
+    st = f.stats()
+    print("lap3d_64 levels fp32_refined", uplo, nrhs, st)
+    assert st["method"] == 2 and st["refine_steps"] == 1
+    X = f.solve(B, nrhs=nrhs).reshape(nrhs, m.n)
+    for j in range(nrhs):
+        assert relerr(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B.reshape(nrhs, m.n)[j])) <= TOL
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
 @pytest.mark.parametrize("nrhs", [1, 5, 11])
 def test_levels_forced_banded(native, precision, nrhs):
