@@ -173,6 +173,17 @@ int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int
 
 int spike_factor_destroy(spike_factor_t f);
 
+/* Device waits of a solve are bounded (the sync-free multi-RHS level solve
+ * and the single-launch Spike solve poll values that other workgroups
+ * publish); a solve whose wait gives up writes NaN where rows could not be
+ * computed.  That outcome is returned as SPIKE_ERR_DEVICE by the call itself
+ * when it synchronises (host-pointer and profiled solves), otherwise by the
+ * next spike_factor_solve* call on the factor or by spike_factor_status,
+ * which the caller may call after synchronising the solve's stream.  Each
+ * give-up is reported once.  No counterpart in the reference, whose solve
+ * has no device waits. */
+int spike_factor_status(spike_factor_t f);
+
 /* Writes a factor (partition maps, block-solve schedules, spikes' reduced
  * system, coupling data, solver choices) to a file, so that a later process
  * can skip preprocessing; spike_factor_load re-creates it in device memory
@@ -296,6 +307,10 @@ typedef struct {
                               /* request; < 0 = 200                            */
     int max_batch_rhs;        /* right-hand sides merged into one solve, and   */
                               /* the most one request may carry; 0 = 8         */
+    double admit_timeout_ms;  /* spike_server_add of a matrix whose footprint  */
+                              /* does not fit the remaining budget waits this  */
+                              /* long for spike_server_remove to free memory;  */
+                              /* < 0 = without limit, 0 = refuse at once       */
 } spike_server_options_t;
 
 typedef struct {
@@ -316,12 +331,21 @@ typedef struct spike_server_s *spike_server_t;
 
 void spike_server_options_default(spike_server_options_t *o);
 int spike_server_create(spike_server_t *s, const spike_server_options_t *o);
-/* Preprocesses a matrix and keeps it resident; fails with SPIKE_ERR_NOMEM
- * when its footprint does not fit the remaining budget.  *id names it. */
+/* Preprocesses a matrix and keeps it resident; *id names it.  Its footprint
+ * (factor + staging) is admitted against the budget: when it does not fit
+ * what is left, the call waits up to admit_timeout_ms for removals to free
+ * memory, then fails with SPIKE_ERR_NOMEM (at once when the footprint alone
+ * exceeds the budget).  May be called while requests are being served. */
 int spike_server_add(spike_server_t s, char uplo, int n, const double *a, const int *ia, const int *ja,
                      const spike_options_t *opts, int *id);
+/* Stops serving matrix id: new submits for it fail with SPIKE_ERR_ARG, the
+ * call waits until its queued and in-flight requests have completed, then
+ * frees its factor and staging and returns its footprint to the budget
+ * (waking adds that wait for memory). */
+int spike_server_remove(spike_server_t s, int id);
 /* Queues a solve of matrix id for nrhs right-hand sides (device pointers,
- * column-major); b and x must stay valid until the ticket completes. */
+ * column-major; host pointers are refused with SPIKE_ERR_ARG); b and x must
+ * stay valid until the ticket completes. */
 int spike_server_submit(spike_server_t s, int id, int nrhs, const double *b, int ldb, double *x, int ldx,
                         uint64_t *ticket);
 /* Blocks until the ticket's solve has completed (x written). */

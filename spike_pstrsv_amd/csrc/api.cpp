@@ -148,6 +148,11 @@ struct spike_factor_s {
     bool sync_free = true;      // multi-RHS: one sync-free launch instead of one per level
     int spin_limit = spike::kSpinLimit;
     int *d_abort = nullptr;     // its abort word
+    // Failure word of the bounded waits: pinned host memory that a kernel
+    // sets (system scope) when a wait gives up; the library reads it after
+    // synchronising solves, at the start of the next call on the factor and
+    // in spike_factor_status.  d_fail is its device address.
+    int *h_fail = nullptr, *d_fail = nullptr;
     double *d_dlt = nullptr;
     int lev_G = 1;
     // interleaved right-hand sides / solutions of multi-RHS level solves
@@ -219,6 +224,8 @@ struct spike_factor_s {
         allocs.clear();
         alloc_bytes.clear();
         device_bytes = 0;
+        if (h_fail) (void)hipHostFree(h_fail);
+        h_fail = d_fail = nullptr;
     }
 };
 
@@ -455,6 +462,30 @@ int upload_row_blocks(spike_factor_s *f, const std::vector<DevPart> &parts)
     return f->upload(&f->d_res, rb);
 }
 
+// The pinned host failure word of the bounded waits (see spike_factor_s).
+int setup_fail_word(spike_factor_s *f)
+{
+    if (f->h_fail) return SPIKE_OK;
+    void *h = nullptr;
+    HIP_TRY(hipHostMalloc(&h, sizeof(int), hipHostMallocMapped));
+    f->h_fail = static_cast<int *>(h);
+    *f->h_fail = 0;
+    void *d = nullptr;
+    HIP_TRY(hipHostGetDevicePointer(&d, h, 0));
+    f->d_fail = static_cast<int *>(d);
+    return SPIKE_OK;
+}
+
+// SPIKE_ERR_DEVICE (once) when a bounded wait of an earlier solve on f gave
+// up, i.e. that solve's x holds NaN where rows could not be computed.
+int take_failure(spike_factor_s *f)
+{
+    if (!f->h_fail || __atomic_load_n(f->h_fail, __ATOMIC_ACQUIRE) == 0) return SPIKE_OK;
+    __atomic_store_n(f->h_fail, 0, __ATOMIC_RELEASE);
+    return fail(SPIKE_ERR_DEVICE, "a solve on this factor gave up a bounded device wait (its x holds NaN); "
+                                  "the device did not run the solve's workgroups in dispatch order");
+}
+
 // Sync-free switch (SPIKE_PSTRSV_SYNC_FREE=0 selects one launch per level)
 // and the abort word it needs (every solve clears it with a kernel).
 int setup_sync_free(spike_factor_s *f)
@@ -465,7 +496,7 @@ int setup_sync_free(spike_factor_s *f)
     if (const char *l = std::getenv("SPIKE_PSTRSV_SPIN_LIMIT")) f->spin_limit = std::max(0, std::atoi(l));
     if (!f->sync_free || f->d_abort) return SPIKE_OK;
     int rc;
-    if ((rc = f->dalloc(&f->d_abort, 1))) return rc;
+    if ((rc = f->dalloc(&f->d_abort, 1)) || (rc = setup_fail_word(f))) return rc;
     HIP_TRY(hipMemset(f->d_abort, 0, sizeof(int)));
     return SPIKE_OK;
 }
@@ -1255,6 +1286,7 @@ spike::LevelArgs level_il_args(const spike_factor_s *f)
     la.invd32 = f->d_invd32;
     la.fp32 = f->fp32 ? 1 : 0;
     la.spin_limit = f->spin_limit;
+    la.fail_word = f->d_fail;
     return la;
 }
 
@@ -1593,6 +1625,8 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
     if (nrhs < 0 || ldb < n || ldx < n) return fail(SPIKE_ERR_ARG, "nrhs must be >= 0 and ldb, ldx >= n");
     if (nrhs == 0) return SPIKE_OK;
     if (!b || !x) return fail(SPIKE_ERR_ARG, "b and x must be non-NULL");
+    // an asynchronous solve before this one that gave up is reported here
+    if (int rc = take_failure(f)) return rc;
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev != f->device) HIP_TRY(hipSetDevice(f->device));
@@ -1656,7 +1690,10 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
         HIP_TRY(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx, sizeof(double) * n, sizeof(double) * n, nrhs,
                                  hipMemcpyDeviceToHost, s));
     }
-    if (!bdev || !xdev) HIP_TRY(hipStreamSynchronize(s));
+    if (!bdev || !xdev || timed) {
+        HIP_TRY(hipStreamSynchronize(s));
+        return take_failure(f);
+    }
     return SPIKE_OK;
 }
 
@@ -2280,6 +2317,12 @@ int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int
     return factor_solve(f, nrhs, b, ldb, x, ldx, (hipStream_t)stream, true, phase_ms);
 }
 
+int spike_factor_status(spike_factor_t f)
+{
+    if (!f) return fail(SPIKE_ERR_ARG, "factor is NULL");
+    return take_failure(f);
+}
+
 int spike_factor_save(spike_factor_t f, const char *path)
 {
     if (!f || !path) return fail(SPIKE_ERR_ARG, "NULL argument");
@@ -2328,22 +2371,25 @@ int spike_factor_log(spike_factor_t f)
 {
     if (!f) return fail(SPIKE_ERR_ARG, "factor is NULL");
     const Plan &pl = f->plan;
-    std::printf("* * * * * * * * * * * * * * * * * * * * * *\n");
-    std::printf("* Matrix Information\n");
-    std::printf("* ------------------\n");
-    std::printf("* Reduced system size: %d x %d\n", f->stats.n_reduced, f->stats.n_reduced);
-    std::printf("* Reduced system nnzs: %lld\n", (long long)f->stats.nnz_reduced);
+    // The reference's text byte for byte (ops.c:1317-1332): the marker lines
+    // end in a space and every field line in tabs, which log parsers match
+    // (the marker string) and split on (the tab after each value).
+    std::printf("* * * * * * * * * * * * * * * * * * * * * * \n");
+    std::printf("* Matrix Information\t\t\t\t\t\t\n");
+    std::printf("* ------------------\t\t\t\t\t\t\n");
+    std::printf("* Reduced system size: %d x %d\t\t\t\t\n", f->stats.n_reduced, f->stats.n_reduced);
+    std::printf("* Reduced system nnzs: %lld \t\t\t\t\t\n", (long long)f->stats.nnz_reduced);
     for (int t = 0; t < pl.P; t++) {
         const spike::Part &pt = pl.parts[t];
-        std::printf("* S[%d] size: %d x %d\n", t, pt.rows_S, pt.kcnt);
-        std::printf("* isOptimized[%d]: %d\n", t, 0);
-        std::printf("* hasReflection[%d]: %d\n", t, pt.dcnt > 0 ? 1 : 0);
-        std::printf("* hasDependences[%d]: %d\n", t, pt.ccnt > 0 ? 1 : 0);
-        std::printf("* hasToCalculateS[%d]: %d\n", t, pt.rows_S > 0 && pt.kcnt > 0 ? 1 : 0);
-        std::printf("* Intervals start[%d]: %d, end[%d]: %d\n", t, pt.f, t, pt.g + 1);
-        std::printf("* Partition start[%d]: %d, end[%d]: %d\n", t, pt.s, t, pt.s + pt.k);
+        std::printf("* S[%d] size: %d x %d\t\t\t\t\t\n", t, pt.rows_S, pt.kcnt);
+        std::printf("* isOptimized[%d]: %d\t\t\t\t\t\n", t, 0);
+        std::printf("* hasReflection[%d]: %d\t\t\t\t\t\n", t, pt.dcnt > 0 ? 1 : 0);
+        std::printf("* hasDependences[%d]: %d\t\t\t\t\n", t, pt.ccnt > 0 ? 1 : 0);
+        std::printf("* hasToCalculateS[%d]: %d\t\t\t\t\n", t, pt.rows_S > 0 && pt.kcnt > 0 ? 1 : 0);
+        std::printf("* Intervals start[%d]: %d, end[%d]: %d\t\n", t, pt.f, t, pt.g + 1);
+        std::printf("* Partition start[%d]: %d, end[%d]: %d\t\n", t, pt.s, t, pt.s + pt.k);
     }
-    std::printf("* * * * * * * * * * * * * * * * * * * * * *\n\n");
+    std::printf("* * * * * * * * * * * * * * * * * * * * * * \n\n");
     std::fflush(stdout);
     return SPIKE_OK;
 }

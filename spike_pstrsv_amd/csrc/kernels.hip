@@ -1152,7 +1152,7 @@ __device__ __forceinline__ bool not_ready(double v)
 // the wait gave up.
 template <int U>
 __device__ __forceinline__ bool poll_values(const double *src, const size_t (&idx)[U], const bool (&in)[U],
-                                            double (&v)[U], int spin_limit, int *abort_word)
+                                            double (&v)[U], int spin_limit, int *abort_word, int *fail_word)
 {
     for (int spins = 0;; spins++) {
 #pragma unroll
@@ -1164,7 +1164,10 @@ __device__ __forceinline__ bool poll_values(const double *src, const size_t (&id
         if (ready) return true;
         if (spins >= spin_limit ||
             ((spins & 63) == 63 && __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the first lane to give up also raises the caller-visible
+            // failure word (host memory, read by the library after the solve)
+            if (__hip_atomic_exchange(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 && fail_word)
+                __hip_atomic_store(fail_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return false;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -1178,7 +1181,7 @@ template <int NRP, class T>
 __global__ __launch_bounds__(256) void k_sf_il(int cnt, const int4 *__restrict__ rec, const int *__restrict__ lja,
                                                const T *__restrict__ lval, const T *__restrict__ invd,
                                                const double *__restrict__ bi, double *xi, int spin_limit,
-                                               int *abort_word)
+                                               int *abort_word, int *fail_word)
 {
     constexpr int U = 4;
     const int gid = blockIdx.x * 256 + threadIdx.x;
@@ -1207,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_sf_il(int cnt, const int4 *__restrict__
             const T wl = lval[ee];
             w[u] = in[u] ? wl : (T)0;
         }
-        ok = poll_values<U>(xi, idx, in, xv, spin_limit, abort_word);
+        ok = poll_values<U>(xi, idx, in, xv, spin_limit, abort_word, fail_word);
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (in[u]) acc = fma(w[u], (T)xv[u], acc);
@@ -1620,10 +1623,10 @@ static hipError_t sf_il_n(int npos, const LevelArgs &a, const double *bi, double
     const dim3 grid((unsigned)(((size_t)npos * NRP + 255) / 256)), block(256);
     if (a.fp32)
         hipLaunchKernelGGL((k_sf_il<NRP, float>), grid, block, 0, s, npos, (const int4 *)a.rec, a.lja, a.lval32,
-                           a.invd32, bi, xi, a.spin_limit, abort_word);
+                           a.invd32, bi, xi, a.spin_limit, abort_word, a.fail_word);
     else
         hipLaunchKernelGGL((k_sf_il<NRP, double>), grid, block, 0, s, npos, (const int4 *)a.rec, a.lja, a.lval,
-                           a.invd, bi, xi, a.spin_limit, abort_word);
+                           a.invd, bi, xi, a.spin_limit, abort_word, a.fail_word);
     return hipGetLastError();
 }
 

@@ -148,6 +148,8 @@ struct LevelArgs {
     double *x;
     int ldb, ldx, nr, nr_slots, n, flip, G, fp32;
     int spin_limit;         // sync-free: polls before a row gives up
+    int *fail_word;         // sync-free: set to 1 (system scope) when a wait
+                            // gives up; host memory mapped for the device
 };
 // Default sync-free poll limit: about a second of polling.
 constexpr int kSpinLimit = 1 << 20;

@@ -1,7 +1,11 @@
 // Serving loop over resident factors (extension API, include/spike_pstrsv.h).
 //
-// Factors stay preprocessed in device memory; a matrix is admitted while the
-// resident footprint (factor + its staging buffers) fits the HBM budget.
+// Factors stay preprocessed in device memory; a matrix is admitted when the
+// resident footprint (factor + its staging buffers) fits the HBM budget, and
+// an add that does not fit waits for removals to free memory.  Per-matrix
+// state lives in heap records that never move, and a launched solve carries
+// copies of what it needs, so matrices can be added and removed while
+// requests are served.
 // Solve requests (device pointers) are queued by spike_server_submit.  A
 // dispatcher thread closes a batching window window_us after the oldest
 // waiting request arrived, merges the window's requests of each matrix into
@@ -27,6 +31,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -53,8 +58,25 @@ struct InFlight {
     int id, sidx;
 };
 
+// One resident matrix.  Records are heap objects owned by the server and
+// never move or disappear while the server lives (removal frees the factor
+// and staging and marks the record), so a pointer to one stays valid.
+struct MatEntry {
+    spike_factor_t f = nullptr;
+    int n = 0;
+    int64_t bytes = 0;
+    double *stage = nullptr;  // b then x, n x max_batch_rhs each
+    int sidx = -1, busy = 0;  // stream of its solves in flight, their count
+    int64_t open = 0;         // requests queued or in flight
+    bool removed = false;
+};
+
+// A merged solve: everything launch() reads is copied here under the lock.
 struct Job {
     int id, cols, sidx;
+    spike_factor_t f;
+    int n;
+    double *stage;
     std::vector<Request> reqs;
     hipEvent_t ev;
 };
@@ -65,19 +87,17 @@ struct spike_server_s {
     spike_server_options_t opts{};
     int device = 0;
     std::mutex mu;
-    std::condition_variable cv_submit, cv_done;
-    std::vector<spike_factor_t> mats;
-    std::vector<int> mat_n;
-    std::vector<int64_t> mat_bytes;
-    std::vector<double *> stage;          // per matrix: b then x, n x max_batch_rhs each
+    std::condition_variable cv_submit, cv_done, cv_admit;
+    std::vector<std::unique_ptr<MatEntry>> mats;
     int64_t resident = 0;
+    int admitting = 0;                    // adds waiting for memory
     std::deque<Request> pending;
     std::vector<InFlight> inflight;
     std::vector<hipStream_t> streams;
     // Stream choice: a matrix with solves in flight (or being launched) keeps
     // their stream, so its solves stay ordered; otherwise it takes the stream
     // with the fewest solves in flight.
-    std::vector<int> mat_sidx, mat_busy, stream_busy;
+    std::vector<int> stream_busy;
     std::vector<hipEvent_t> spare_events;
     uint64_t next_ticket = 1;
     std::map<uint64_t, int> finished;     // ticket -> status (until collected)
@@ -111,6 +131,16 @@ int sfail(int code, const std::string &m)
     return code;
 }
 
+bool device_ptr(const void *p)
+{
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.isManaged;
+}
+
 }  // namespace
 
 // Polls in-flight solves and retires the finished ones (mu held).
@@ -129,6 +159,7 @@ void spike_server_s::poll_done(std::unique_lock<std::mutex> &lk)
             finished[r.ticket] = st;
             lat.push_back(secs(r.t_submit, now));
         }
+        mats[inflight[i].id]->open -= (int64_t)inflight[i].reqs.size();
         rows_rhs_done += inflight[i].rows_rhs;
         release_stream(inflight[i].id, inflight[i].sidx);
         t_last = now;
@@ -142,30 +173,32 @@ void spike_server_s::poll_done(std::unique_lock<std::mutex> &lk)
 // Stream of a new solve of matrix id (mu held); counted until released.
 int spike_server_s::pick_stream(int id)
 {
-    int sidx = mat_busy[id] > 0 ? mat_sidx[id] : -1;
+    MatEntry &m = *mats[id];
+    int sidx = m.busy > 0 ? m.sidx : -1;
     if (sidx < 0) {
         sidx = 0;
         for (int k = 1; k < (int)streams.size(); k++)
             if (stream_busy[k] < stream_busy[sidx]) sidx = k;
     }
-    mat_sidx[id] = sidx;
-    mat_busy[id]++;
+    m.sidx = sidx;
+    m.busy++;
     stream_busy[sidx]++;
     return sidx;
 }
 
 void spike_server_s::release_stream(int id, int sidx)
 {
-    mat_busy[id]--;
+    mats[id]->busy--;
     stream_busy[sidx]--;
 }
 
 // Launches one merged solve (mu not held): gather, solve, scatter, event.
+// Reads only the job and the immutable options and stream pool.
 int spike_server_s::launch(const Job &j)
 {
-    const int n = mat_n[j.id];
+    const int n = j.n;
     hipStream_t s = streams[j.sidx];
-    double *mb = stage[j.id], *mx = stage[j.id] + (size_t)n * opts.max_batch_rhs;
+    double *mb = j.stage, *mx = j.stage + (size_t)n * opts.max_batch_rhs;
     // every column of the merged requests moves in one copy launch (per
     // spike::kMaxCopyCols columns) each way
     auto copy_cols = [&](bool in) {
@@ -189,7 +222,7 @@ int spike_server_s::launch(const Job &j)
     int padded = 1;
     while (padded < j.cols) padded <<= 1;
     padded = std::min(padded, opts.max_batch_rhs);
-    int rc = spike_factor_solve(mats[j.id], padded, mb, n, mx, n, s);
+    int rc = spike_factor_solve(j.f, padded, mb, n, mx, n, s);
     if (rc != SPIKE_OK) return rc;
     if (!copy_cols(false)) return SPIKE_ERR_DEVICE;
     return hipEventRecord(j.ev, s) == hipSuccess ? SPIKE_OK : SPIKE_ERR_DEVICE;
@@ -237,6 +270,9 @@ void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
             Job j;
             j.id = kv.first;
             j.cols = 0;
+            j.f = mats[j.id]->f;
+            j.n = mats[j.id]->n;
+            j.stage = mats[j.id]->stage;
             while (i < rs.size() && j.cols + rs[i].nrhs <= opts.max_batch_rhs) {
                 j.cols += rs[i].nrhs;
                 j.reqs.push_back(rs[i++]);
@@ -275,6 +311,7 @@ void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
             (void)hipStreamSynchronize(streams[j.sidx]);
             release_stream(j.id, j.sidx);
             for (const Request &r : j.reqs) finished[r.ticket] = rcs[k];
+            mats[j.id]->open -= (int64_t)j.reqs.size();
             spare_events.push_back(j.ev);
             cv_done.notify_all();
             continue;
@@ -282,7 +319,7 @@ void spike_server_s::dispatch_window(std::unique_lock<std::mutex> &lk)
         InFlight fl;
         fl.done = j.ev;
         fl.reqs = std::move(j.reqs);
-        fl.rows_rhs = (int64_t)mat_n[j.id] * j.cols;
+        fl.rows_rhs = (int64_t)j.n * j.cols;
         fl.id = j.id;
         fl.sidx = j.sidx;
         inflight.push_back(std::move(fl));
@@ -341,6 +378,7 @@ void spike_server_options_default(spike_server_options_t *o)
     o->streams = 8;
     o->window_us = 200.0;
     o->max_batch_rhs = 8;
+    o->admit_timeout_ms = -1.0;
 }
 
 int spike_server_create(spike_server_t *out, const spike_server_options_t *opts)
@@ -387,52 +425,101 @@ int spike_server_add(spike_server_t s, char uplo, int n, const double *a, const 
     if (fo) fopt = *fo;
     else spike_options_default(&fopt);
     if (fopt.use_graph == 1) fopt.use_graph = 2;
+    const int64_t stage_bytes = (int64_t)n * s->opts.max_batch_rhs * 2 * (int64_t)sizeof(double);
     spike_factor_t f = nullptr;
     int rc = spike_factor_create(&f, uplo, n, a, ia, ja, &fopt);
     if (rc != SPIKE_OK) return sfail(rc, spike_last_error());
     spike_stats_t st;
     spike_factor_stats(f, &st);
-    const int64_t stage_bytes = (int64_t)n * s->opts.max_batch_rhs * 2 * (int64_t)sizeof(double);
-    const int64_t need = st.device_bytes + stage_bytes;
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->resident + need > (int64_t)s->opts.hbm_budget) {
+    int64_t need = st.device_bytes + stage_bytes;
+    if (need > (int64_t)s->opts.hbm_budget) {
         spike_factor_destroy(f);
-        return sfail(SPIKE_ERR_NOMEM, "matrix footprint of " + std::to_string(need) + " bytes exceeds the HBM budget (" +
-                                          std::to_string((int64_t)s->opts.hbm_budget - s->resident) + " bytes left)");
+        return sfail(SPIKE_ERR_NOMEM, "matrix footprint of " + std::to_string(need) +
+                                          " bytes exceeds the whole HBM budget (" +
+                                          std::to_string((int64_t)s->opts.hbm_budget) + " bytes)");
     }
+    // Staging and the warm-up solves (which capture the factor's solve graph
+    // for every padded column count) run before the matrix is visible to the
+    // dispatcher, on a stream of their own, without the server lock.
     double *stg = nullptr;
     if (hipMalloc((void **)&stg, (size_t)stage_bytes) != hipSuccess) {
         (void)hipGetLastError();
         spike_factor_destroy(f);
         return sfail(SPIKE_ERR_NOMEM, "staging allocation failed");
     }
-    // zeroed staging; then every padded column count is solved twice so the
-    // factor captures its solve graphs now rather than while serving
-    if (hipMemset(stg, 0, (size_t)stage_bytes) != hipSuccess) {
+    hipStream_t ws = nullptr;
+    if (hipMemset(stg, 0, (size_t)stage_bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(stg);
         spike_factor_destroy(f);
-        return sfail(SPIKE_ERR_DEVICE, "staging clear failed");
+        return sfail(SPIKE_ERR_DEVICE, "staging set-up failed");
     }
     {
-        hipStream_t ws = s->streams[s->mats.size() % s->streams.size()];
         double *mb = stg, *mx = stg + (size_t)n * s->opts.max_batch_rhs;
         for (int c = 1; c <= s->opts.max_batch_rhs && rc == SPIKE_OK; c <<= 1)
             for (int rep = 0; rep < 2 && rc == SPIKE_OK; rep++) rc = spike_factor_solve(f, c, mb, n, mx, n, ws);
         if (rc == SPIKE_OK && hipStreamSynchronize(ws) != hipSuccess) rc = SPIKE_ERR_DEVICE;
+        if (rc == SPIKE_OK) rc = spike_factor_status(f);
+        (void)hipStreamDestroy(ws);
+        // the footprint includes workspaces the warm-up solves allocated
+        spike_factor_stats(f, &st);
+        need = st.device_bytes + stage_bytes;
         if (rc != SPIKE_OK) {
             (void)hipFree(stg);
             spike_factor_destroy(f);
             return sfail(rc, "warm-up solve failed");
         }
     }
+    // admission: wait for the budget (removals notify cv_admit)
+    std::unique_lock<std::mutex> lk(s->mu);
+    const double tmo = s->opts.admit_timeout_ms;
+    const clk::time_point deadline =
+        clk::now() + std::chrono::microseconds((int64_t)(std::max(tmo, 0.0) * 1e3));
+    s->admitting++;
+    while (s->resident + need > (int64_t)s->opts.hbm_budget && !s->stop.load()) {
+        if (tmo == 0.0 || (tmo > 0.0 && clk::now() >= deadline)) break;
+        s->cv_admit.wait_for(lk, std::chrono::milliseconds(2));
+    }
+    s->admitting--;
+    if (s->resident + need > (int64_t)s->opts.hbm_budget || s->stop.load()) {
+        const int64_t left = (int64_t)s->opts.hbm_budget - s->resident;
+        lk.unlock();
+        (void)hipFree(stg);
+        spike_factor_destroy(f);
+        return sfail(SPIKE_ERR_NOMEM, "matrix footprint of " + std::to_string(need) +
+                                          " bytes exceeds the HBM budget left (" + std::to_string(left) +
+                                          " bytes)");
+    }
+    auto m = std::make_unique<MatEntry>();
+    m->f = f;
+    m->n = n;
+    m->bytes = need;
+    m->stage = stg;
     s->resident += need;
-    s->mats.push_back(f);
-    s->mat_n.push_back(n);
-    s->mat_sidx.push_back(-1);
-    s->mat_busy.push_back(0);
-    s->mat_bytes.push_back(need);
-    s->stage.push_back(stg);
+    s->mats.push_back(std::move(m));
     *id = (int)s->mats.size() - 1;
+    return SPIKE_OK;
+}
+
+int spike_server_remove(spike_server_t s, int id)
+{
+    if (!s) return sfail(SPIKE_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int)s->mats.size() || s->mats[id]->removed) return sfail(SPIKE_ERR_ARG, "unknown matrix id");
+    MatEntry &m = *s->mats[id];
+    m.removed = true;   // no new requests
+    while (m.open > 0) s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+    spike_factor_t f = m.f;
+    double *stg = m.stage;
+    m.f = nullptr;
+    m.stage = nullptr;
+    lk.unlock();
+    spike_factor_destroy(f);
+    (void)hipFree(stg);
+    lk.lock();
+    s->resident -= m.bytes;
+    m.bytes = 0;
+    s->cv_admit.notify_all();
     return SPIKE_OK;
 }
 
@@ -440,11 +527,15 @@ int spike_server_submit(spike_server_t s, int id, int nrhs, const double *b, int
                         uint64_t *ticket)
 {
     if (!s || !ticket || !b || !x) return sfail(SPIKE_ERR_ARG, "NULL argument");
+    // the copy kernels read b and write x directly: host memory would fault
+    if (!device_ptr(b) || !device_ptr(x)) return sfail(SPIKE_ERR_ARG, "b and x must be device pointers");
     Request r;
     r.t_submit = clk::now();
     std::lock_guard<std::mutex> lk(s->mu);
-    if (id < 0 || id >= (int)s->mats.size()) return sfail(SPIKE_ERR_ARG, "unknown matrix id");
-    if (nrhs < 1 || nrhs > s->opts.max_batch_rhs || ldb < s->mat_n[id] || ldx < s->mat_n[id])
+    if (id < 0 || id >= (int)s->mats.size() || s->mats[id]->removed)
+        return sfail(SPIKE_ERR_ARG, "unknown matrix id");
+    const int n = s->mats[id]->n;
+    if (nrhs < 1 || nrhs > s->opts.max_batch_rhs || ldb < n || ldx < n)
         return sfail(SPIKE_ERR_ARG, "nrhs must be in [1, max_batch_rhs] and ldb, ldx >= n");
     if (s->error) return sfail(s->error, "server failed: " + s->error_msg);
     r.ticket = s->next_ticket++;
@@ -459,6 +550,7 @@ int spike_server_submit(spike_server_t s, int id, int nrhs, const double *b, int
         s->have_first = true;
     }
     s->pending.push_back(r);
+    s->mats[id]->open++;
     *ticket = r.ticket;
     s->cv_submit.notify_one();
     return SPIKE_OK;
@@ -495,7 +587,7 @@ int spike_server_stats(spike_server_t s, spike_server_stats_t *st)
     if (!s || !st) return sfail(SPIKE_ERR_ARG, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     std::memset(st, 0, sizeof *st);
-    st->matrices = (int)s->mats.size();
+    for (const auto &m : s->mats) st->matrices += m->removed ? 0 : 1;
     st->resident_bytes = s->resident;
     st->completed = (int64_t)s->lat.size();
     st->solves = s->solves;
@@ -534,11 +626,19 @@ int spike_server_destroy(spike_server_t s)
     if (!s) return SPIKE_OK;
     s->stop.store(true);
     s->cv_submit.notify_all();
+    s->cv_admit.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    {
+        // adds still waiting for memory see stop and fail
+        std::unique_lock<std::mutex> lk(s->mu);
+        while (s->admitting > 0) s->cv_admit.wait_for(lk, std::chrono::milliseconds(1));
+    }
     for (hipStream_t t : s->streams) (void)hipStreamDestroy(t);
     for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
-    for (spike_factor_t f : s->mats) spike_factor_destroy(f);
-    for (double *p : s->stage) (void)hipFree(p);
+    for (auto &m : s->mats) {
+        if (m->f) spike_factor_destroy(m->f);
+        if (m->stage) (void)hipFree(m->stage);
+    }
     delete s;
     return SPIKE_OK;
 }

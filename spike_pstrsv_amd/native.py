@@ -25,13 +25,14 @@ ERRORS = {1: "SPIKE_ERR_ARG", 2: "SPIKE_ERR_MATRIX", 3: "SPIKE_ERR_DEVICE", 4: "
 EXPORTED = [
     "spike_pstrsv_preproc", "_spike_pstrsv_preproc", "spike_pstrsv", "spike_pstrsv_solve", "_spike_pstrsv_solve",
     "spike_options_default", "spike_factor_create", "spike_factor_create_partitioned", "spike_factor_solve",
-    "spike_factor_solve_profiled",
+    "spike_factor_solve_profiled", "spike_factor_status",
     "spike_factor_destroy",
     "spike_factor_stats", "spike_factor_log", "spike_analyze", "spike_pstrsv_set_partitions",
     "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
     "spike_batch_options_default", "spike_batch_create", "spike_batch_offset", "spike_batch_length",
     "spike_batch_solve", "spike_batch_stats", "spike_batch_destroy",
-    "spike_server_options_default", "spike_server_create", "spike_server_add", "spike_server_submit",
+    "spike_server_options_default", "spike_server_create", "spike_server_add", "spike_server_remove",
+    "spike_server_submit",
     "spike_server_wait", "spike_server_drain", "spike_server_stats", "spike_server_reset_stats",
     "spike_server_destroy", "spike_server_last_error",
     "spike_read_matrix_market", "spike_free", "spike_mm_last_error",
@@ -87,7 +88,7 @@ class BatchStats(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("slots", C.c_int), ("streams", C.c_int), ("window_us", C.c_double),
-                ("max_batch_rhs", C.c_int)]
+                ("max_batch_rhs", C.c_int), ("admit_timeout_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -129,6 +130,8 @@ lib.spike_factor_solve.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
 lib.spike_factor_solve.restype = _i
 lib.spike_factor_solve_profiled.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp, C.POINTER(_d)]
 lib.spike_factor_solve_profiled.restype = _i
+lib.spike_factor_status.argtypes = [_vp]
+lib.spike_factor_status.restype = _i
 lib.spike_factor_destroy.argtypes = [_vp]
 lib.spike_factor_destroy.restype = _i
 lib.spike_factor_stats.argtypes = [_vp, C.POINTER(Stats)]
@@ -164,6 +167,8 @@ lib.spike_server_create.argtypes = [C.POINTER(_vp), C.POINTER(ServerOptions)]
 lib.spike_server_create.restype = _i
 lib.spike_server_add.argtypes = [_vp, _c, _i, _vp, _vp, _vp, C.POINTER(Options), C.POINTER(_i)]
 lib.spike_server_add.restype = _i
+lib.spike_server_remove.argtypes = [_vp, _i]
+lib.spike_server_remove.restype = _i
 lib.spike_server_submit.argtypes = [_vp, _i, _i, _vp, _i, _vp, _i, C.POINTER(C.c_uint64)]
 lib.spike_server_submit.restype = _i
 lib.spike_server_wait.argtypes = [_vp, C.c_uint64]
@@ -282,6 +287,10 @@ class Factor:
         _check(lib.spike_factor_solve_profiled(self.handle, nrhs, _ptr(b), n if ldb is None else ldb, _ptr(x),
                                                n if ldx is None else ldx, stream or None, ms))
         return list(ms)
+
+    def status(self) -> None:
+        """Raises SpikeError (SPIKE_ERR_DEVICE) if an earlier solve gave up a bounded device wait."""
+        _check(lib.spike_factor_status(self.handle))
 
     def save(self, path) -> None:
         _check(lib.spike_factor_save(self.handle, str(path).encode()))
@@ -430,7 +439,7 @@ class Server:
     Requests carry device pointers (int addresses or tensors with data_ptr())."""
 
     def __init__(self, hbm_budget: int = 0, slots: int = 64, streams: int = 8, window_us: float = 200.0,
-                 max_batch_rhs: int = 8):
+                 max_batch_rhs: int = 8, admit_timeout_ms: float = -1.0):
         o = ServerOptions()
         lib.spike_server_options_default(C.byref(o))
         o.hbm_budget = hbm_budget
@@ -438,6 +447,7 @@ class Server:
         o.streams = streams
         o.window_us = window_us
         o.max_batch_rhs = max_batch_rhs
+        o.admit_timeout_ms = admit_timeout_ms
         h = _vp()
         _scheck(lib.spike_server_create(C.byref(h), C.byref(o)))
         self.handle = h.value
@@ -449,8 +459,14 @@ class Server:
         mid = C.c_int(-1)
         _scheck(lib.spike_server_add(self.handle, uplo.encode()[:1], len(ia) - 1, a.ctypes.data, ia.ctypes.data,
                                      ja.ctypes.data, C.byref(o), C.byref(mid)))
-        self.n.append(len(ia) - 1)
+        while len(self.n) <= mid.value:
+            self.n.append(0)
+        self.n[mid.value] = len(ia) - 1
         return mid.value
+
+    def remove(self, mid: int) -> None:
+        """Stops serving matrix mid once its requests complete and frees its memory."""
+        _scheck(lib.spike_server_remove(self.handle, mid))
 
     def submit(self, mid: int, nrhs: int, b, x, ldb: int | None = None, ldx: int | None = None) -> int:
         n = self.n[mid]

@@ -187,17 +187,33 @@ def test_sync_free_matches_level_launches(native, monkeypatch, uplo, precision):
         assert relerr(X1[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])) <= TOL
 
 
-def test_sync_free_give_up_is_bounded(native, monkeypatch):
+def test_sync_free_give_up_is_reported(native, monkeypatch):
     """With a poll limit of 0 (forced from the host) rows that find an
-    operand unpublished give up at once: the solve returns, the rows that
-    gave up and everything after them read NaN, and every finite value is
-    the exact result."""
+    operand unpublished give up at once.  The solve still ends, every finite
+    value is the exact result, and the give-up is returned as
+    SPIKE_ERR_DEVICE: by the call itself when it synchronises (host
+    pointers), by spike_factor_status after an asynchronous (device-pointer)
+    solve; each give-up is reported once."""
+    import torch
+
     m = gen.lap3d(32)
     B = gen.rhs(m.n, 8, seed=gen.SEED_BASE + 181)
     want = native.Factor("L", m.a, m.ia, m.ja, method="levels").solve(B, nrhs=8)
     monkeypatch.setenv("SPIKE_PSTRSV_SPIN_LIMIT", "0")
     f = native.Factor("L", m.a, m.ia, m.ja, method="levels")
-    X = f.solve(B, nrhs=8)
+    X = np.zeros((8, m.n))
+    with pytest.raises(native.SpikeError) as ei:
+        f.solve(B, X, nrhs=8)
+    assert ei.value.code == 3, ei.value
     fin = np.isfinite(X)
     assert not fin.all(), "the give-up path did not run"
     assert np.array_equal(X[fin], want[fin])
+    f.status()                                   # reported once
+    db = torch.from_numpy(np.ascontiguousarray(B)).to("cuda:0")
+    dx = torch.zeros_like(db)
+    f.solve(db.data_ptr(), dx.data_ptr(), nrhs=8)   # asynchronous: returns
+    torch.cuda.synchronize()
+    with pytest.raises(native.SpikeError) as ei:
+        f.status()
+    assert ei.value.code == 3, ei.value
+    f.status()

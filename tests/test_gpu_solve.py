@@ -281,3 +281,59 @@ def test_graph_replays_match_direct_launches(native, method, use_graph):
         assert np.array_equal(dx.cpu().numpy(), want), f"call {call}"
     for j in range(3):
         assert relerr(want[j], orc.ldsol(m.a, m.ia, m.ja, B[j])) <= TOL
+
+
+def _parse_partition_log(text, nthreads):
+    """The reference's log parser, restated: tools/plotSpeedUp.py:186-198
+    finds the block between two marker lines by the marker string (with its
+    trailing space), and :68-84 reads the fields, cutting each at its tab."""
+    marker = "* * * * * * * * * * * * * * * * * * * * * * "
+    block, inside = [], None
+    for line in text.splitlines(keepends=True):
+        if marker in line:
+            if inside is None:
+                inside = True
+            else:
+                break
+        elif inside:
+            block.append(line)
+    assert inside, "no marker line"
+    t = 0
+    while "Matrix Information" not in block[t]:
+        t += 1
+    content = block[t + 2:]
+    k = int(content[0][23:].split(" x ")[0])
+    nnz = int(content[1].split(": ")[1])
+    per = []
+    for th in range(nthreads):
+        s = 1 + th * 7
+        field = lambda i: content[s + i].split("\t")[0].split(": ")[1]  # noqa: E731
+        iv = content[s + 6].split(", ")
+        per.append({"S": field(1), "isOptimized": field(2) == "1", "hasReflection": field(3) == "1",
+                    "hasDependence": field(4) == "1", "hasToCalculateS": field(5) == "1",
+                    "start": int(iv[0].split(": ")[-1]), "end": int(iv[1].split("\t")[0].split(": ")[1])})
+    return k, nnz, per
+
+
+def test_partition_log_parses_like_the_reference_tool(native, lap2d, capfd):
+    """spike_factor_log prints the reference's logPSTRSV3 text byte for byte
+    (ops.c:1317-1332: marker lines ending in a space, field lines ending in
+    tabs), so the reference's parser reads it."""
+    m = lap2d
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=8)
+    capfd.readouterr()
+    f.log()
+    out = capfd.readouterr().out
+    lines = out.splitlines(keepends=True)
+    assert lines[0] == "* * * * * * * * * * * * * * * * * * * * * * \n"
+    assert lines[1] == "* Matrix Information\t\t\t\t\t\t\n"
+    assert lines[3].endswith("\t\t\t\t\n") and lines[4].endswith(" \t\t\t\t\t\n")
+    st = f.stats()
+    k, nnz, per = _parse_partition_log(out, st["partitions"])
+    assert k == st["n_reduced"] and nnz == st["nnz_reduced"]
+    assert len(per) == 8
+    assert per[0]["hasDependence"] is False and all(p["hasDependence"] for p in per[1:])
+    assert all(p["hasReflection"] for p in per[:-1]) and not per[-1]["hasReflection"]
+    for p in per:
+        r, c = (int(v) for v in p["S"].split(" x "))
+        assert p["hasToCalculateS"] == (r > 0 and c > 0)
