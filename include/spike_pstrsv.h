@@ -218,6 +218,12 @@ typedef struct {
                               /* not computed)                                 */
     int rhs_tile[4];          /* right-hand sides per block-solve wave at 1,   */
                               /* 2, 4, 8 slots (Spike method)                  */
+    int single_launch;        /* 1: a Spike solve of up to 8 right-hand sides  */
+                              /* is one kernel launch (pass 1, the Jacobi      */
+                              /* sweeps and pass 2 as items of one persistent  */
+                              /* grid; FP64 factors, opt-in with the           */
+                              /* environment SPIKE_PSTRSV_FUSED=1); 0: a       */
+                              /* launch sequence                               */
 } spike_stats_t;
 
 int spike_factor_stats(spike_factor_t f, spike_stats_t *st);

@@ -153,6 +153,18 @@ struct spike_factor_s {
     // synchronising solves, at the start of the next call on the factor and
     // in spike_factor_status.  d_fail is its device address.
     int *h_fail = nullptr, *d_fail = nullptr;
+    // Single-launch solve (spike::launch_spike_fused): its sweep items (int4
+    // {first row, end row, first entry, end entry} of the reduced system)
+    // and, per RHS-slot class (1, 2, 4, 8), its published vectors and queue
+    // counters, allocated on first use outside any graph capture
+    // (prepare_fused); fused_fit caches per slot class and tile whether the
+    // launch fits the device (-1 unknown).
+    bool fused = false;
+    int *d_chunks = nullptr;
+    int nchunks = 0;
+    double *d_fx[4] = {};
+    unsigned *d_fctr[4] = {};
+    signed char fused_fit[4][4] = {{-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}, {-1, -1, -1, -1}};
     double *d_dlt = nullptr;
     int lev_G = 1;
     // interleaved right-hand sides / solutions of multi-RHS level solves
@@ -365,6 +377,7 @@ void choose_rhs_slots(spike_factor_s *f)
 int calibrate_refinement(spike_factor_s *f);
 int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
                hipEvent_t *ev, double *ph);
+int prepare_fused(spike_factor_s *f, int nr);
 
 // Autotuner of the RHS tile: for 2, 4 and 8 slots, every tile that fits the
 // LDS is timed on zero right-hand sides (two warm-up solves, then the fastest
@@ -385,6 +398,7 @@ int tune_tiles(spike_factor_s *f)
     HIP_TRY(hipEventCreate(&e1));
     for (int li = 1; li < 4 && (1 << li) <= f->max_nr && !rc; li++) {
         const int c = 1 << li;
+        if ((rc = prepare_fused(f, c))) break;
         int best_t = f->tile_at[li];
         float best_ms = 1e30f;
         for (int ti = 0; ti <= li && !rc; ti++) {
@@ -486,14 +500,20 @@ int take_failure(spike_factor_s *f)
                                   "the device did not run the solve's workgroups in dispatch order");
 }
 
+// Poll limit of the bounded device waits; SPIKE_PSTRSV_SPIN_LIMIT (a test
+// hook: a small limit forces the give-up path) overrides the default.
+void read_spin_limit(spike_factor_s *f)
+{
+    if (const char *l = std::getenv("SPIKE_PSTRSV_SPIN_LIMIT")) f->spin_limit = std::max(0, std::atoi(l));
+}
+
 // Sync-free switch (SPIKE_PSTRSV_SYNC_FREE=0 selects one launch per level)
 // and the abort word it needs (every solve clears it with a kernel).
 int setup_sync_free(spike_factor_s *f)
 {
     const char *e = std::getenv("SPIKE_PSTRSV_SYNC_FREE");
     f->sync_free = !(e && std::atoi(e) == 0);
-    // test hook: a small poll limit forces the give-up path
-    if (const char *l = std::getenv("SPIKE_PSTRSV_SPIN_LIMIT")) f->spin_limit = std::max(0, std::atoi(l));
+    read_spin_limit(f);
     if (!f->sync_free || f->d_abort) return SPIKE_OK;
     int rc;
     if ((rc = f->dalloc(&f->d_abort, 1)) || (rc = setup_fail_word(f))) return rc;
@@ -913,6 +933,42 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         if ((rc = f->upload(&f->d_sweep_rows, lists))) return rc;
     }
 
+    // Single-launch solve: with Jacobi sweeps (or a reduced system without
+    // off-diagonals) and the pass-2 coupling products in LDS.  Sweep items are
+    // runs of consecutive reduced rows of about rnnz / (4 x CUs) entries (at
+    // most 2048, one batch of a wave at one RHS) and kFusedRowsPerItem rows.
+    // Opt-in (SPIKE_PSTRSV_FUSED=1): measured slower than the launch sequence
+    // on a 1M-row banded matrix (0.29 against 0.13 ms per solve), and one run
+    // of an FP32 factor ended in a device memory fault whose cause is not
+    // identified; the launch sequence stays the default.
+    {
+        const char *e = std::getenv("SPIKE_PSTRSV_FUSED");
+        f->fused = e && std::atoi(e) == 1 && nred > 0 && (f->sweeps > 0 || rnnz == 0) && f->cpl_fused &&
+                   !pl.steps.fwd && !f->fp32;
+        if (f->fused) {
+            const int64_t target =
+                std::min<int64_t>(2048, std::max<int64_t>(256, (rnnz + 4LL * f->ncu - 1) / (4LL * f->ncu)));
+            std::vector<int> ch;
+            for (int q = 0; q < nred;) {
+                const int q0 = q;
+                int64_t ent = 0;
+                while (q < nred && q - q0 < spike::kFusedRowsPerItem &&
+                       (q == q0 || ent + (hrp[q + 1] - hrp[q]) <= target)) {
+                    ent += hrp[q + 1] - hrp[q];
+                    q++;
+                }
+                ch.insert(ch.end(), {q0, q, hrp[q0], hrp[q]});
+            }
+            f->nchunks = (int)(ch.size() / 4);
+            read_spin_limit(f);
+            if ((rc = f->upload(&f->d_chunks, ch)) || (rc = setup_fail_word(f))) return rc;
+            if (!f->d_abort) {
+                if ((rc = f->dalloc(&f->d_abort, 1))) return rc;
+                HIP_TRY(hipMemset(f->d_abort, 0, sizeof(int)));
+            }
+        }
+    }
+
     // host copies no longer needed
     pl.a.clear(); pl.a.shrink_to_fit();
     pl.ja.clear(); pl.ja.shrink_to_fit();
@@ -940,6 +996,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     st.levels = f->nlev_full;
     if (f->batch_slots == 0 && f->opts.autotune && !f->steps.fwd && (rc = tune_tiles(f))) return rc;
     for (int li = 0; li < 4; li++) st.rhs_tile[li] = f->tile_at[li];
+    st.single_launch = f->fused ? 1 : 0;
     if (f->fp32) {
         if ((rc = upload_row_blocks(f, hp))) return rc;
         if ((rc = calibrate_refinement(f))) return rc;
@@ -959,14 +1016,14 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 // File layout: magic, a block of scalars, then tagged records
 // {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
 // stored by content; workspaces only by size (re-allocated on load).
-constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '4'};
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '5'};
 
 struct FactorScalars {
     spike_options_t opts;
     spike_stats_t stats;
     char uplo;
     int flip, n, P, kmax, nlev, red_in_lds, sweeps, ring, cpl_fused, fuse_at[4], tile_at[4], max_nr, cmax;
-    int fp32, refine_steps, method, caller_parts, nlev_full, lev_G, nres, G, M, R, fwd;
+    int fp32, refine_steps, method, caller_parts, nlev_full, lev_G, nres, G, M, R, fwd, fused, nchunks;
     int64_t nnz, spike_area;
     double refine_rate, refine_est;
     uint64_t lds_needed;
@@ -990,7 +1047,7 @@ std::vector<std::pair<const char *, void **>> device_fields(spike_factor_s *f)
             {"ja__", (void **)&f->d_ja},             {"lrec", (void **)&f->d_lrec},
             {"lja_", (void **)&f->d_lja},            {"lval", (void **)&f->d_lval},
             {"lv32", (void **)&f->d_lval32},         {"id32", (void **)&f->d_invd32},
-            {"res_", (void **)&f->d_res}};
+            {"res_", (void **)&f->d_res},            {"fchk", (void **)&f->d_chunks}};
 }
 std::vector<std::pair<const char *, double **>> workspace_fields(spike_factor_s *f)
 {
@@ -1053,6 +1110,8 @@ int factor_save(spike_factor_s *f, const char *path)
     sc.M = f->steps.M;
     sc.R = f->steps.R;
     sc.fwd = f->steps.fwd;
+    sc.fused = f->fused;
+    sc.nchunks = f->nchunks;
     sc.nnz = pl.nnz;
     sc.spike_area = pl.spike_area;
     sc.refine_rate = f->refine_rate;
@@ -1150,6 +1209,8 @@ int factor_load(spike_factor_s *f, const char *path)
     f->steps.M = sc.M;
     f->steps.R = sc.R;
     f->steps.fwd = sc.fwd;
+    f->fused = sc.fused != 0;
+    f->nchunks = sc.nchunks;
     (void)hipDeviceGetAttribute(&f->ncu, hipDeviceAttributeMultiprocessorCount, f->device);
     (void)hipDeviceGetAttribute(&f->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, f->device);
     (void)hipGetLastError();
@@ -1199,6 +1260,12 @@ int factor_load(spike_factor_s *f, const char *path)
     }
     if (f->method == SPIKE_METHOD_LEVELS)
         if (int rc = setup_sync_free(f)) return rc;
+    if (f->fused) {
+        read_spin_limit(f);
+        if (int rc = setup_fail_word(f)) return rc;
+        if (int rc = f->dalloc(&f->d_abort, 1)) return rc;
+        HIP_TRY(hipMemset(f->d_abort, 0, sizeof(int)));
+    }
     f->stats.device_bytes = f->device_bytes;
     return SPIKE_OK;
 }
@@ -1359,6 +1426,40 @@ int solve_levels(spike_factor_s *f, const double *b, int ldb, double *x, int ldx
     return SPIKE_OK;
 }
 
+// Workspace of the single-launch solve for solves of nr right-hand sides:
+// the slot class's published vectors (two parities of S+1 vectors of
+// nred x slots values) filled with the not-ready pattern, and its counters.
+// Called before any launch that may be captured into a graph.
+int prepare_fused(spike_factor_s *f, int nr)
+{
+    if (!f->fused || nr < 1) return SPIKE_OK;
+    int slots = f->batch_slots > 0 ? f->batch_slots : 1;
+    while (slots < nr) slots <<= 1;
+    const int li = __builtin_ctz((unsigned)slots);
+    if (li > 3 || f->d_fx[li]) return SPIKE_OK;
+    const size_t len = 2 * (size_t)(f->sweeps + 1) * f->plan.dep_rows.size() * (size_t)slots;
+    int rc;
+    if ((rc = f->dalloc(&f->d_fx[li], len)) || (rc = f->dalloc(&f->d_fctr[li], 96))) return rc;
+    HIP_TRY(spike::launch_fused_reset(f->d_fx[li], len, f->d_fctr[li], f->d_abort, nullptr));
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    return SPIKE_OK;
+}
+
+// Whether the single-launch solve serves these launch arguments (workspace
+// prepared, and the kernel fits the device at this tile).
+bool use_fused(spike_factor_s *f, const spike::BlockArgs &ba)
+{
+    if (!f->fused || f->plan.dep_rows.empty()) return false;
+    const int li = __builtin_ctz((unsigned)ba.nr_slots), ti = __builtin_ctz((unsigned)ba.tile);
+    if (li > 3 || ti > 3 || !f->d_fx[li]) return false;
+    if (f->fused_fit[li][ti] < 0) {
+        spike::BlockArgs q = ba;
+        q.cpl_fused = 1;
+        f->fused_fit[li][ti] = spike::spike_fused_lds(q) > 0 ? 1 : 0;
+    }
+    return f->fused_fit[li][ti] == 1;
+}
+
 int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, int accum, hipStream_t s,
                hipEvent_t *ev, double *ph)
 {
@@ -1366,6 +1467,28 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
     const Plan &pl = f->plan;
     spike::BlockArgs ba = make_args(f, b, ldb, x, ldx, nr, accum);
     const int slots = ba.nr_slots;
+    if (!ev && use_fused(f, ba)) {
+        // one launch: pass 1, the Jacobi sweeps and pass 2 with its coupling
+        // update (profiled solves keep the launch sequence, whose phases can
+        // be timed apart)
+        const int li = __builtin_ctz((unsigned)slots);
+        ba.cpl_fused = 1;
+        spike::FusedArgs fa{};
+        fa.X = f->d_fx[li];
+        fa.ctr = f->d_fctr[li];
+        fa.chunks = f->d_chunks;
+        fa.nchunks = f->nchunks;
+        fa.nred = (int)pl.dep_rows.size();
+        fa.S = f->sweeps;
+        fa.rp = f->d_rrp;
+        fa.ci = f->d_rci;
+        fa.v = f->d_rv;
+        fa.spin_limit = f->spin_limit;
+        fa.abort_word = f->d_abort;
+        fa.fail_word = f->d_fail;
+        HIP_TRY(spike::launch_spike_fused(pl.P, ba, fa, f->ncu, s));
+        return SPIKE_OK;
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[0], s));
     HIP_TRY(spike::launch_block_solve(1, pl.P, ba, s));
     if (ev) HIP_TRY(hipEventRecord(ev[1], s));
@@ -1504,6 +1627,7 @@ int calibrate_refinement(spike_factor_s *f)
     if ((rc = f->dalloc(&db, len)) || (rc = f->dalloc(&dx, len))) return rc;
     HIP_TRY(hipMemcpy(db, hb.data(), sizeof(double) * len, hipMemcpyHostToDevice));
     f->refine_steps = 0;
+    if ((rc = prepare_fused(f, nr))) return rc;
     if ((rc = solve_pass(f, db, n, dx, n, nr, 0, nullptr, nullptr, nullptr))) return rc;
     HIP_TRY(hipMemcpy(hprev.data(), dx, sizeof(double) * len, hipMemcpyDeviceToHost));
     std::vector<double> c;
@@ -1633,6 +1757,8 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
     const bool bdev = is_device_ptr(b), xdev = is_device_ptr(x);
     {
         int rc = ensure_level_ws(f, nrhs);
+        if (!rc) rc = prepare_fused(f, std::min(nrhs, f->max_nr));
+        if (!rc && nrhs > f->max_nr && nrhs % f->max_nr) rc = prepare_fused(f, nrhs % f->max_nr);
         if (rc) return rc;
     }
     if (bdev && xdev && !timed && f->opts.use_graph > 0 &&
@@ -1987,6 +2113,7 @@ int batch_build(spike_batch_s *bt, const spike_batch_matrix_t *m, const std::vec
                              &f);
         if (rc) return rc;
         bt->groups.push_back(f);
+        if ((rc = prepare_fused(f, g.slots))) return rc;
     }
     if (bt->opts.precision == SPIKE_PREC_FP32_REFINED && !bt->groups.empty()) {
         // one residual workspace for all groups: their matrices own disjoint

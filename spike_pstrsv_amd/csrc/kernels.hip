@@ -16,6 +16,8 @@
 // are bit-for-bit reproducible.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace spike {
@@ -203,6 +205,74 @@ __device__ __forceinline__ void load_rhs(const BlockArgs &A, const RhsMap &m, T 
     }
 }
 
+// Values published by other workgroups during a launch (the sync-free level
+// solve, the fused Spike solve) carry their readiness: the vector starts
+// filled with kNotReady, a signalling-NaN pattern that arithmetic never
+// produces (results of NaN operands are quiet NaNs); each value is published
+// by one 64-bit agent-scope store and read by polling with agent-scope loads,
+// which are performed at the agent's coherence point (the XCDs' L2 caches are
+// not coherent with each other).
+constexpr unsigned long long kNotReady = 0x7FF4DEADBEEF0001ull;
+
+__device__ __forceinline__ double load_agent(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool not_ready(double v)
+{
+    return (unsigned long long)__double_as_longlong(v) == kNotReady;
+}
+
+// Polls v[u] = src[idx[u]] (u with in[u]) until none is kNotReady; false if
+// the wait gave up.
+template <int U, class I>
+__device__ __forceinline__ bool poll_values(const double *src, const I (&idx)[U], const bool (&in)[U],
+                                            double (&v)[U], int spin_limit, int *abort_word, int *fail_word)
+{
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = load_agent(&src[idx[u]]);   // all polls in flight together
+    for (int spins = 0, nap = 1;; spins++) {
+        bool ready = true;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u] && not_ready(v[u])) ready = false;
+        if (ready) return true;
+        if (spins >= spin_limit ||
+            ((spins & 63) == 63 && __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            // the first lane to give up also raises the caller-visible
+            // failure word (host memory, read by the library after the solve)
+            if (__hip_atomic_exchange(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 && fail_word)
+                __hip_atomic_store(fail_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (not_ready(v[u])) v[u] = __builtin_nan("");   // never pass the pattern on
+            return false;
+        }
+        // back off (up to about 1000 cycles), then poll again only what is
+        // still unpublished, so waiting waves do not flood the memory system
+        for (int z = 0; z < nap; z++) __builtin_amdgcn_s_sleep(4);
+        nap = nap < 4 ? nap * 2 : 4;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u] && not_ready(v[u])) v[u] = load_agent(&src[idx[u]]);
+    }
+}
+
+// The fused Spike solve's view of its published vectors (one launch, see
+// FusedArgs): xcur holds x^(0) .. x^(S) of this solve's parity, mstride values
+// apart (nred x nr_slots each), xnext the other parity's, which this solve
+// refills with kNotReady for the next one.
+struct FusedCtx {
+    double *xcur, *xnext;
+    size_t mstride;
+    int S, spin_limit;
+    int *abort_word, *fail_word;
+};
+
 // Pass-2 right-hand side of one partition in LDS, NR right-hand sides
 // interleaved (xs[i*NR + r]): b for every row, then b - sum_j c_ij xr[col_j]
 // on the coupling rows (times 1/diag when scaled).  The coupling update is
@@ -211,9 +281,9 @@ __device__ __forceinline__ void load_rhs(const BlockArgs &A, const RhsMap &m, T 
 // entry order, exactly as k_coupling does, so the result depends neither on
 // the lane count nor on which of the two computes it.  Right-hand sides
 // r >= nr reuse column nr-1.
-template <int NR, class T>
-__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt, T *xs, double *prod, bool scaled,
-                                          int roff, int RS)
+template <int NR, class T, bool FUSED>
+__device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const FusedCtx &F, const DevPart &pt, T *xs,
+                                          double *prod, bool scaled, int roff, int RS)
 {
     const int lane = threadIdx.x, s = pt.s, k = pt.k;
     const RhsMap m = rhs_map(A, pt);
@@ -264,10 +334,30 @@ __device__ __forceinline__ void pass2_rhs(const BlockArgs &A, const DevPart &pt,
                 w[u] = cv[ee];
                 col[u] = A.cpl_col[ee];
             }
+            if constexpr (FUSED) {
+                // x_K of the last sweep, published by other workgroups
+                unsigned idx[UC * NR];
+                bool in[UC * NR];
+                double pv[UC * NR];
 #pragma unroll
-            for (int u = 0; u < UC; u++)
+                for (int u = 0; u < UC; u++)
 #pragma unroll
-                for (int r = 0; r < NR; r++) xv[u][r] = A.xr[(size_t)col[u] * RS + roff + r];
+                    for (int r = 0; r < NR; r++) {
+                        idx[u * NR + r] = (unsigned)col[u] * (unsigned)RS + (unsigned)(roff + r);
+                        in[u * NR + r] = e + 64 * u < e1;
+                    }
+                (void)poll_values<UC * NR>(F.xcur + (size_t)F.S * F.mstride, idx, in, pv, F.spin_limit,
+                                           F.abort_word, F.fail_word);
+#pragma unroll
+                for (int u = 0; u < UC; u++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) xv[u][r] = pv[u * NR + r];
+            } else {
+#pragma unroll
+                for (int u = 0; u < UC; u++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) xv[u][r] = A.xr[(size_t)col[u] * RS + roff + r];
+            }
 #pragma unroll
             for (int u = 0; u < UC; u++) {   // all loads before the conditional stores
                 asm volatile("" ::"v"(w[u]));
@@ -344,13 +434,19 @@ struct SlotVec<T, 8> {
 // nr_slots right-hand sides runs nr_slots / NR waves per partition
 // (blockIdx.y picks columns roff .. roff+NR-1); the reduced vectors hold
 // nr_slots interleaved values per index.
-template <int G, int M, int D, int PASS, int NR, class T>
-__global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
+//
+// block_pass is the body of one (partition p, RHS tile yb) wave; with FUSED
+// it runs as a work item of the single-launch solve (k_spike_fused): pass 1
+// publishes y at the dependents into x^(0) of the current parity (and
+// refills the other parity's vectors of its dependents for the next solve),
+// pass 2 polls x^(S) of the coupling columns instead of reading a reduced
+// vector written by an earlier launch.
+template <int G, int M, int D, int PASS, int NR, class T, bool FUSED>
+__device__ __forceinline__ void block_pass(const BlockArgs &A, const FusedCtx &F, int p, int yb,
+                                           unsigned char *smem_blk)
 {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem_blk[];
     T *xs = reinterpret_cast<T *>(smem_blk);
     constexpr int R = 64 / G;
-    const int p = blockIdx.x;
     const DevPart pt = A.parts[p];
     // the step range is loaded with the partition record (one round trip
     // for both, not one after the other around the early returns)
@@ -364,7 +460,7 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     if (PASS == 1 && pt.hi1 < 0) return;
     if (PASS == 2 && pt.f >= k) return;
     const RhsMap m = rhs_map(A, pt);
-    const int roff = blockIdx.y * NR, RS = A.nr_slots;
+    const int roff = yb * NR, RS = A.nr_slots;
 
     constexpr unsigned TB = sizeof(T);
     const int nst = (int)(se - sb);
@@ -415,7 +511,8 @@ __global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
     if (PASS == 1) {
         load_rhs<NR, T>(A, m, xs, s, k, roff, true);
     } else {
-        pass2_rhs<NR, T>(A, pt, xs, reinterpret_cast<double *>(smem_blk + xs_bytes<T>(A.kmax, NR)), true, roff, RS);
+        pass2_rhs<NR, T, FUSED>(A, F, pt, xs, reinterpret_cast<double *>(smem_blk + xs_bytes<T>(A.kmax, NR)), true,
+                                roff, RS);
     }
     if (lane == 0)
 #pragma unroll
@@ -524,7 +621,11 @@ epilogue:
             for (int u = 0; u < UP; u++) v[u] = xs[ir0[u]];
 #pragma unroll
             for (int u = 0; u < UP; u++)
-                if (lane + 64 * u < pt.dcnt) A.xred[(size_t)(pt.dbeg + lane + 64 * u) * RS + roff] = (double)v[u];
+                if (lane + 64 * u < pt.dcnt) {
+                    const size_t at = (size_t)(pt.dbeg + lane + 64 * u) * RS + roff;
+                    if constexpr (FUSED) store_agent(F.xcur + at, (double)v[u]);
+                    else A.xred[at] = (double)v[u];
+                }
             q = 64 * UP;
         }
         // the rest (several RHS, or more dependents): indices loaded per trip
@@ -544,8 +645,21 @@ epilogue:
             for (int u = 0; u < UE; u++)
                 if (q0 + 64 * u < pt.dcnt)
 #pragma unroll
-                    for (int r = 0; r < NR; r++)
-                        A.xred[(size_t)(pt.dbeg + q0 + 64 * u) * RS + roff + r] = (double)v[u][r];
+                    for (int r = 0; r < NR; r++) {
+                        const size_t at = (size_t)(pt.dbeg + q0 + 64 * u) * RS + roff + r;
+                        if constexpr (FUSED) store_agent(F.xcur + at, (double)v[u][r]);
+                        else A.xred[at] = (double)v[u][r];
+                    }
+        }
+        if constexpr (FUSED) {
+            // the next solve (other parity) starts with every value of these
+            // dependents unpublished: nothing reads that parity during this
+            // solve, and the launch boundary orders these stores before it
+            const int nv = pt.dcnt * NR;
+            for (int mm = 0; mm <= F.S; mm++)
+                for (int i = lane; i < nv; i += 64)
+                    store_agent(F.xnext + (size_t)mm * F.mstride + (size_t)(pt.dbeg + i / NR) * RS + roff + i % NR,
+                                __longlong_as_double((long long)kNotReady));
         }
     }
     // pass 2 writes the solution; without coupling, pass 1's result is final
@@ -596,6 +710,186 @@ epilogue:
     }
 }
 
+template <int G, int M, int D, int PASS, int NR, class T>
+__global__ __launch_bounds__(64) void k_block_solve(BlockArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_blk[];
+    block_pass<G, M, D, PASS, NR, T, false>(A, FusedCtx{}, blockIdx.x, blockIdx.y, smem_blk);
+}
+
+// One sweep item of the fused solve: x^(m) = y - W x^(m-1) on the reduced
+// rows [ch.x, ch.y) (entries [ch.z, ch.w) of the reduced CSR) for the NR
+// right-hand sides roff .. roff+NR-1 (nr_slots = RS values per index).  The
+// entries are taken 64*UW at a time (UW*NR = 32 values per lane): values and
+// columns, then the polled x^(m-1) values; the products go to the LDS
+// scratch prod.  Each row's products of the batch are then summed by a group
+// of 8 lanes (lane li adds entries li, li+8, ... in order, then a fixed DPP
+// tree), and the batch sums are added in batch order to the row's sum in
+// rowacc; the result depends neither on NR nor on the lane that runs the row.
+// y (the row's own x^(0)) is loaded with the row bounds and polled again only
+// if it was not yet published then.  LDS: prod (64*32 doubles), rowacc
+// (kFusedRowsPerItem*NR doubles), the row bounds (kFusedRowsPerItem+1 ints).
+template <int NR>
+__device__ __forceinline__ void sweep_item(const FusedArgs &FA, const FusedCtx &F, int4 ch, int m, int roff, int RS,
+                                           unsigned char *lds)
+{
+    constexpr int UW = 32 / NR >= 1 ? 32 / NR : 1;
+    constexpr int VR = kFusedRowsPerItem / 64;
+    double *prod = reinterpret_cast<double *>(lds);
+    double *rowacc = prod + 64 * 32;
+    int *rbs = reinterpret_cast<int *>(rowacc + kFusedRowsPerItem * NR);
+    const int lane = threadIdx.x, g = lane >> 3, li = lane & 7;
+    const int nrows = ch.y - ch.x;
+    const double *xprev = F.xcur + (size_t)(m - 1) * F.mstride;
+    double *xout = F.xcur + (size_t)m * F.mstride;
+    unsigned yidx[VR * NR];
+    bool yin[VR * NR];
+    double yv[VR * NR];
+#pragma unroll
+    for (int v = 0; v < VR; v++) {
+        const int j = lane + 64 * v;
+        const bool ok = j < nrows;
+        const int qc = ch.x + (ok ? j : nrows - 1);
+        const int rb = FA.rp[qc];
+        if (ok) rbs[j] = rb;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            yidx[v * NR + r] = (unsigned)qc * (unsigned)RS + (unsigned)(roff + r);
+            yin[v * NR + r] = ok;
+            yv[v * NR + r] = load_agent(F.xcur + yidx[v * NR + r]);
+        }
+    }
+    if (lane == 0) rbs[nrows] = ch.w;
+    for (int eb = ch.z; eb < ch.w; eb += 64 * UW) {
+        unsigned idx[UW * NR];
+        bool in[UW * NR];
+        double w[UW], xv[UW * NR];
+#pragma unroll
+        for (int u = 0; u < UW; u++) {
+            const int e = eb + lane + 64 * u;
+            const bool ok = e < ch.w;
+            const int ee = ok ? e : ch.w - 1;
+            const int c = FA.ci[ee];
+            const double wv = FA.v[ee];
+            w[u] = ok ? wv : 0.0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                idx[u * NR + r] = (unsigned)c * (unsigned)RS + (unsigned)(roff + r);
+                in[u * NR + r] = ok;
+            }
+        }
+        (void)poll_values<UW * NR>(xprev, idx, in, xv, F.spin_limit, F.abort_word, F.fail_word);
+        __syncthreads();   // the previous batch's sums have read prod (and rbs is written)
+#pragma unroll
+        for (int u = 0; u < UW; u++)
+#pragma unroll
+            for (int r = 0; r < NR; r++) prod[(lane + 64 * u) * NR + r] = w[u] * xv[u * NR + r];
+        __syncthreads();
+        const int ee = min(ch.w, eb + 64 * UW);
+        for (int j = g; j < nrows; j += 8) {
+            const int lo = max(rbs[j], eb), hi = min(rbs[j + 1], ee);
+            double part[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) part[r] = 0.0;
+            for (int e = lo + li; e < hi; e += 32) {
+                double t4[4][NR];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) t4[u][r] = e + 8 * u < hi ? prod[(e + 8 * u - eb) * NR + r] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) part[r] += t4[u][r];
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                part[r] = group_sum<8>(part[r]);
+                if (li == 0) rowacc[j * NR + r] = (eb == ch.z ? 0.0 : rowacc[j * NR + r]) + part[r];
+            }
+        }
+    }
+    __syncthreads();
+    bool yready = true;
+#pragma unroll
+    for (int q = 0; q < VR * NR; q++)
+        if (yin[q] && not_ready(yv[q])) yready = false;
+    if (!yready) (void)poll_values<VR * NR>(F.xcur, yidx, yin, yv, F.spin_limit, F.abort_word, F.fail_word);
+#pragma unroll
+    for (int v = 0; v < VR; v++)
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+            if (yin[v * NR + r]) {
+                const int j = lane + 64 * v;
+                const double sum = ch.w > ch.z ? rowacc[j * NR + r] : 0.0;
+                store_agent(xout + yidx[v * NR + r], yv[v * NR + r] - sum);
+            }
+}
+
+// Single-launch Spike solve (FusedArgs): a persistent grid whose workgroups
+// (one wave each) take tickets from the in-order queue until it is empty.
+// Ticket order: pass-1 items (partition p, RHS tile), sweep items of sweep 1
+// .. S (chunk, tile), pass-2 items.  An item polls only values published by
+// items of smaller tickets, which running workgroups hold, so the lowest
+// unfinished item never waits for an unfinished one.  The next ticket is
+// taken when an item starts, so its round trip overlaps the item.
+template <int G, int M, int D, int NR, class T>
+__global__ __launch_bounds__(64) void k_spike_fused(BlockArgs A, FusedArgs FA, int P)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_f[];
+    const int lane = threadIdx.x;
+    const int RS = A.nr_slots, tiles = RS / NR;
+    unsigned *c_par = FA.ctr, *c_queue = FA.ctr + 32, *c_exit = FA.ctr + 64;
+    const unsigned par = __hip_atomic_load(c_par, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u;
+    FusedCtx F;
+    F.mstride = (size_t)FA.nred * RS;
+    const size_t half = (size_t)(FA.S + 1) * F.mstride;
+    F.xcur = FA.X + (size_t)par * half;
+    F.xnext = FA.X + (size_t)(par ^ 1u) * half;
+    F.S = FA.S;
+    F.spin_limit = FA.spin_limit;
+    F.abort_word = FA.abort_word;
+    F.fail_word = FA.fail_word;
+    const unsigned np = (unsigned)P * tiles, nc = (unsigned)FA.nchunks * tiles;
+    const unsigned p2 = np + (unsigned)FA.S * nc, total = p2 + np;
+    unsigned raw = 0;
+    if (lane == 0) raw = __hip_atomic_fetch_add(c_queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned t = (unsigned)__builtin_amdgcn_readfirstlane((int)raw);
+    while (t < total) {
+        if (lane == 0) raw = __hip_atomic_fetch_add(c_queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t < np) {
+            block_pass<G, M, D, 1, NR, T, true>(A, F, (int)(t % (unsigned)P), (int)(t / (unsigned)P), smem_f);
+        } else if (t < p2) {
+            const unsigned u = t - np, w = u % nc;
+            const int4 ch = reinterpret_cast<const int4 *>(FA.chunks)[w % (unsigned)FA.nchunks];
+            sweep_item<NR>(FA, F, ch, 1 + (int)(u / nc), (int)(w / (unsigned)FA.nchunks) * NR, RS, smem_f);
+        } else {
+            const unsigned u = t - p2;
+            block_pass<G, M, D, 2, NR, T, true>(A, F, (int)(u % (unsigned)P), (int)(u / (unsigned)P), smem_f);
+        }
+        __syncthreads();   // the next item may overwrite what this one left in LDS
+        t = (unsigned)__builtin_amdgcn_readfirstlane((int)raw);
+    }
+    // The last workgroup to leave resets the queue and the abort word and
+    // flips the parity: every workgroup read the parity before its exit
+    // increment, and nothing of this launch touches the counters afterwards.
+    if (lane == 0 && __hip_atomic_fetch_add(c_exit, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+        __hip_atomic_store(c_queue, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(c_exit, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(FA.abort_word, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(c_par, par ^ 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// X[0 .. len) = kNotReady; ctr[0 .. 96) = 0; *abort_word = 0.
+__global__ __launch_bounds__(256) void k_fused_reset(size_t len, double *X, unsigned *ctr, int *abort_word)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < len) X[i] = __longlong_as_double((long long)kNotReady);
+    if (i < 96) ctr[i] = 0u;
+    if (i == 0) *abort_word = 0;
+}
+
 // Forwarding block solve (16 lanes per row slot, 4 slots per step).  Values
 // are pre-scaled by 1/diag; xs[r] starts as b_r/d_r.  Step s of a partition:
 //   E_s  = xs[row] - sum_early val * xs[col]   (lanes of the slot, DPP sum)
@@ -626,7 +920,7 @@ __global__ __launch_bounds__(64) void k_block_solve_fwd(BlockArgs A)
             xs[r] = m.bv(0, g) * invd[g];
         }
     } else {
-        pass2_rhs<1, double>(A, pt, xs, xs + A.kmax + XS_PAD, true, 0, 1);
+        pass2_rhs<1, double, false>(A, FusedCtx{}, pt, xs, xs + A.kmax + XS_PAD, true, 0, 1);
     }
     if (lane == 0) {
         xs[k] = 0.0;
@@ -1133,46 +1427,6 @@ __global__ __launch_bounds__(256) void k_level_il(int r0, int cnt, const int4 *_
 // shows as NaN results instead of a hang; once the abort word is set every
 // waiting row gives up at once.  (The limit is a launch argument so that a
 // test can force the give-up path with a tiny one.)
-constexpr unsigned long long kNotReady = 0x7FF4DEADBEEF0001ull;
-
-__device__ __forceinline__ double load_agent(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_agent(double *p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ bool not_ready(double v)
-{
-    return (unsigned long long)__double_as_longlong(v) == kNotReady;
-}
-
-// Polls v[u] = src[idx[u]] (u with in[u]) until none is kNotReady; false if
-// the wait gave up.
-template <int U>
-__device__ __forceinline__ bool poll_values(const double *src, const size_t (&idx)[U], const bool (&in)[U],
-                                            double (&v)[U], int spin_limit, int *abort_word, int *fail_word)
-{
-    for (int spins = 0;; spins++) {
-#pragma unroll
-        for (int u = 0; u < U; u++) v[u] = load_agent(&src[idx[u]]);   // all polls in flight together
-        bool ready = true;
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (in[u] && not_ready(v[u])) ready = false;
-        if (ready) return true;
-        if (spins >= spin_limit ||
-            ((spins & 63) == 63 && __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            // the first lane to give up also raises the caller-visible
-            // failure word (host memory, read by the library after the solve)
-            if (__hip_atomic_exchange(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 && fail_word)
-                __hip_atomic_store(fail_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
 
 // Interleaved right-hand sides (v[i*NRP + r]), NRP lanes per row as in
 // k_level_il and with its arithmetic (same bits); each lane polls its own
@@ -1449,12 +1703,107 @@ __global__ __launch_bounds__(256) void k_reduced_fill(int nred, const int64_t *_
     }
 }
 
+// Dynamic LDS of the fused solve: the block solve's x segment and its pass-2
+// coupling products, or a sweep item's scratch (products, row sums, row
+// bounds), whichever is larger.
+template <int NR, class T>
+size_t fused_lds_t(const BlockArgs &a)
+{
+    const size_t blk = xs_bytes<T>(a.kmax, NR) + (size_t)a.cmax * NR * sizeof(double);
+    const size_t swp = (size_t)(64 * 32 + kFusedRowsPerItem * NR) * sizeof(double) + (kFusedRowsPerItem + 1) * sizeof(int);
+    return std::max(blk, swp);
+}
+
+template <int G, int D, int NR, class T>
+hipError_t fused_gdt(int P, const BlockArgs &a, const FusedArgs &f, int ncu, hipStream_t s, int *per_cu_out)
+{
+    const size_t lds = fused_lds_t<NR, T>(a);
+    auto kern = k_spike_fused<G, 1, D, NR, T>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu_out) {   // query only
+        *per_cu_out = per_cu;
+        return hipSuccess;
+    }
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    const long tiles = a.nr_slots / NR;
+    const long want = std::max((long)P * tiles, (long)f.nchunks * tiles);
+    const long grid = std::max(1L, std::min((long)per_cu * ncu, want));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64), lds, s, a, f, P);
+    return hipGetLastError();
+}
+
+template <int G>
+hipError_t fused_g(int P, const BlockArgs &a, const FusedArgs &f, int ncu, hipStream_t s, int *q)
+{
+    if (a.tile < 1 || a.tile > a.nr_slots || a.nr_slots % a.tile) return hipErrorInvalidValue;
+#define SPIKE_FUSED_T(NR, D)                                                                            \
+    return a.fp32 ? fused_gdt<G, D, NR, float>(P, a, f, ncu, s, q) : fused_gdt<G, D, NR, double>(P, a, f, ncu, s, q)
+    switch (a.tile) {
+    case 1: SPIKE_FUSED_T(1, 16);
+    case 2: SPIKE_FUSED_T(2, 16);
+    case 4: SPIKE_FUSED_T(4, 8);
+    case 8: SPIKE_FUSED_T(8, 8);
+    default: return hipErrorInvalidValue;
+    }
+#undef SPIKE_FUSED_T
+}
+
+hipError_t fused_dispatch(int P, const BlockArgs &a, const FusedArgs &f, int ncu, hipStream_t s, int *q)
+{
+    if (a.st.fwd) return hipErrorInvalidValue;
+    switch (a.st.G) {
+    case 1: return fused_g<1>(P, a, f, ncu, s, q);
+    case 2: return fused_g<2>(P, a, f, ncu, s, q);
+    case 4: return fused_g<4>(P, a, f, ncu, s, q);
+    case 8: return fused_g<8>(P, a, f, ncu, s, q);
+    case 16: return fused_g<16>(P, a, f, ncu, s, q);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s)
 {
     if (P <= 0) return hipSuccess;
     return pass == 1 ? launch_pass<1>(P, a, s) : launch_pass<2>(P, a, s);
+}
+
+size_t spike_fused_lds(const BlockArgs &a)
+{
+    if (a.st.fwd) return 0;
+    int per_cu = 0;
+    FusedArgs f{};
+    if (fused_dispatch(1, a, f, 1, nullptr, &per_cu) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    switch (a.tile) {
+    case 1: return a.fp32 ? fused_lds_t<1, float>(a) : fused_lds_t<1, double>(a);
+    case 2: return a.fp32 ? fused_lds_t<2, float>(a) : fused_lds_t<2, double>(a);
+    case 4: return a.fp32 ? fused_lds_t<4, float>(a) : fused_lds_t<4, double>(a);
+    case 8: return a.fp32 ? fused_lds_t<8, float>(a) : fused_lds_t<8, double>(a);
+    default: return 0;
+    }
+}
+
+hipError_t launch_spike_fused(int P, const BlockArgs &a, const FusedArgs &f, int ncu, hipStream_t s)
+{
+    if (P <= 0) return hipSuccess;
+    return fused_dispatch(P, a, f, ncu, s, nullptr);
+}
+
+hipError_t launch_fused_reset(double *X, size_t len, unsigned *ctr, int *abort_word, hipStream_t s)
+{
+    const size_t n = std::max<size_t>(len, 96);
+    hipLaunchKernelGGL(k_fused_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, len, X, ctr, abort_word);
+    return hipGetLastError();
 }
 
 template <bool IN_LDS, int NR>

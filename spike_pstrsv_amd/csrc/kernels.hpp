@@ -85,6 +85,44 @@ struct BlockArgs {
 // right-hand side), one wavefront per partition.
 hipError_t launch_block_solve(int pass, int P, const BlockArgs &a, hipStream_t s);
 
+// One launch for a whole Spike solve (pass 1, S Jacobi sweeps of the reduced
+// system, pass 2), as a persistent grid that takes work items from an
+// in-order queue: first the pass-1 items (one per partition and RHS tile),
+// then for every sweep the sweep items (contiguous reduced rows of about
+// equal entry counts), then the pass-2 items.  An item only waits for
+// values that items taken before it publish, so the grid makes progress
+// whatever number of its workgroups is resident.  Published values carry
+// their readiness: X holds two buffers (solve parity) of S+1 vectors of
+// nred x nr_slots values, x^(m)[q*nr_slots + r] (x^(0) = y at the
+// dependents); the buffer of the current parity starts filled with a
+// not-ready pattern, every value is published by one agent-scope store and
+// read by polling, and each solve refills the other parity's buffer for the
+// next.  Waits are bounded (spin_limit polls): an item that gives up publishes
+// NaN, raises *abort_word and sets *fail_word (host memory).  The queue
+// counters, the parity and the exit count live in ctr[0], ctr[32], ctr[64]
+// (separate cache lines); the last workgroup to leave resets the queue and
+// flips the parity, so the launch can be replayed from a graph.
+struct FusedArgs {
+    double *X;
+    unsigned *ctr;
+    const int *chunks;      // int4 per sweep item: first row, end row, first entry, end entry
+    int nchunks;
+    int nred, S;
+    const int *rp, *ci;     // reduced system (unit lower, diagonal implicit): CSR
+    const double *v;        // over reduced rows, ci = reduced column
+    int spin_limit;
+    int *abort_word, *fail_word;
+};
+// Bytes of dynamic LDS the fused solve needs for these arguments, or 0 when
+// it cannot run (LDS or occupancy).
+size_t spike_fused_lds(const BlockArgs &a);
+hipError_t launch_spike_fused(int P, const BlockArgs &a, const FusedArgs &f, int ncu, hipStream_t s);
+// X[0 .. len) = not-ready pattern and the counters (96 words) cleared; run
+// once when a fused workspace is allocated, never inside a captured graph.
+hipError_t launch_fused_reset(double *X, size_t len, unsigned *ctr, int *abort_word, hipStream_t s);
+// Sweep items rows per item at most (4 per lane).
+constexpr int kFusedRowsPerItem = 256;
+
 // Single-workgroup forward solve of the unit lower reduced system, in place on
 // xred.  rlist holds the rows with off-diagonals, grouped by level;
 // levptr[nlev+1] (device) delimits the levels.

@@ -55,7 +55,7 @@ class Stats(C.Structure):
                 ("reduced_sweeps", C.c_int), ("jacobi_bound", C.c_double),
                 ("rhs_per_launch", C.c_int), ("precision", C.c_int), ("refine_steps", C.c_int),
                 ("refine_rate", C.c_double), ("refine_est_err", C.c_double), ("method", C.c_int),
-                ("levels", C.c_int), ("rhs_tile", C.c_int * 4)]
+                ("levels", C.c_int), ("rhs_tile", C.c_int * 4), ("single_launch", C.c_int)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_}
