@@ -59,13 +59,37 @@ def inblock_nnz(m, P: int) -> int:
     return int(np.sum(m.ja >= part * k))
 
 
+def host_cpu():
+    """(threads to use, nproc, CPU model): min(nproc, 128) threads, capped by
+    the CPUs this process may run on and by OMP_NUM_THREADS when it is set."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = nproc
+    threads = min(nproc, 128, allowed)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return max(1, threads), nproc, model
+
+
 def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
-    """Times the oracle's Spike restatement (OpenMP) on the host."""
+    """Times the oracle's Spike restatement (OpenMP, threads pinned close to
+    cores) on the host."""
     import numpy as np
 
     from oracle import pyoracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, nproc, model = host_cpu()
     sp = orc.Spike("L", m.a, m.ia, m.ja, P, tau=0.0, nthreads=threads)
     x = sp.solve(b)
     x = sp.solve(b)
@@ -88,8 +112,25 @@ def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
         "kind": "port",
         "sample": f"{count} solves in {el:.2f} s of the OpenMP Spike restatement (P={P}, same matrix and RHS) "
                   f"after 2 warm-up solves; sequential FP64 substitution alone: {m.n / t_seq:.3e} rows/s on 1 core",
+        "nproc": nproc,
+        "cpu_model": model,
+        "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")},
     }
     return cpu, rel
+
+
+def oracle_err(m, B, X, uplo="L"):
+    """Largest relative error of the columns X[j] against the oracle's
+    sequential FP64 substitution of B[j] (host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import pyoracle as orc
+
+    B = B.reshape(-1, m.n)
+    X = X.reshape(-1, m.n)
+    with ThreadPoolExecutor(host_cpu()[0]) as ex:
+        errs = ex.map(lambda j: reldiff(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])), range(B.shape[0]))
+        return max(errs)
 
 
 def reldiff(x, ref):
@@ -113,6 +154,10 @@ def timed_solves(f, B, X, nr, stream, reps):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def precisions(args):
+    return ("fp64", "fp32_refined") if args.precision == "both" else (args.precision,)
 
 
 def emit(out):
@@ -139,16 +184,17 @@ def bench_lap3d(args):
     B = torch.from_numpy(gen.rhs(n, nr, seed=gen.SEED_BASE + 3).reshape(nr, n)).cuda()
     res = {}
     x64 = None
-    for prec in ("fp64", "fp32_refined"):
+    for prec in precisions(args):
         f = native.Factor("L", m.a, m.ia, m.ja, precision=prec)
         st = f.stats()
         X = torch.zeros_like(B)
         ms = timed_solves(f, B, X, nr, stream, max(3, args.steps // 20))
         Xh = X.cpu().numpy()
-        if x64 is None:
+        if prec == "fp64":
             x64 = Xh
         res[prec] = {"ms_per_solve": ms, "rows_rhs_per_s": n * nr / (ms * 1e-3),
-                     "rel_diff_vs_fp64_gpu": reldiff(Xh, x64),
+                     "rel_err_vs_oracle": oracle_err(m, B.cpu().numpy(), Xh),
+                     "rel_diff_vs_fp64_gpu": reldiff(Xh, x64) if x64 is not None and prec != "fp64" else None,
                      "method": "levels" if st["method"] == 2 else "spike", "levels": st["levels"],
                      "refine_steps": st["refine_steps"], "preproc_s": st["preproc_seconds"]}
         f.close()
@@ -173,7 +219,7 @@ def bench_batched(args):
     stream = torch.cuda.current_stream()
     res = {}
     x64 = None
-    for prec in ("fp64", "fp32_refined"):
+    for prec in precisions(args):
         t0 = time.perf_counter()
         bt = native.Batch("L", ms_in, precision=prec)
         create_s = time.perf_counter() - t0
@@ -196,14 +242,16 @@ def bench_batched(args):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         x = dx.cpu().numpy()
-        if x64 is None:
+        worst = None
+        if prec == "fp64":
             x64 = x
-        worst = 0.0
-        for i in range(len(specs)):
-            seg = slice(bt.offsets[i], bt.offsets[i + 1])
-            worst = max(worst, reldiff(x[seg], x64[seg]))
+        elif x64 is not None:
+            worst = max(reldiff(x[bt.offsets[i]:bt.offsets[i + 1]], x64[bt.offsets[i]:bt.offsets[i + 1]])
+                        for i in range(len(specs)))
+        err = batch_oracle_err(specs, mats, bt.offsets, b, x)
         st = bt.stats()
-        res[prec] = {"ms_per_solve": ms, "rows_rhs_per_s": L / (ms * 1e-3), "rel_diff_vs_fp64_gpu": worst,
+        res[prec] = {"ms_per_solve": ms, "rows_rhs_per_s": L / (ms * 1e-3), "rel_err_vs_oracle": err,
+                     "rel_diff_vs_fp64_gpu": worst,
                      "create_s": create_s, "rows_per_partition": st["rows_per_partition"],
                      "refine_steps": st["refine_steps"], "groups": st["groups"], "graph_nodes": st["graph_nodes"]}
         bt.close()
@@ -219,7 +267,27 @@ def bench_batched(args):
                      "rows_rhs": rows_rhs})
     out["precisions"] = res
     out["solve_roofline_frac"] = b_alg / (best["ms_per_solve"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    # the target's ratio: batched FP32 + refinement over the plain FP64 path
+    if len(res) == 2:
+        out["fp32_refined_over_fp64"] = res["fp32_refined"]["rows_rhs_per_s"] / res["fp64"]["rows_rhs_per_s"]
     return emit(out)
+
+
+def batch_oracle_err(specs, mats, offsets, b, x):
+    """Largest relative error over every column of every matrix of a batch
+    against the oracle (host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import pyoracle as orc
+
+    jobs = []
+    for i, (s, m) in enumerate(zip(specs, mats)):
+        n, r = s["n"], s["nrhs"]
+        Bi = b[offsets[i]:offsets[i + 1]].reshape(r, n)
+        Xi = x[offsets[i]:offsets[i + 1]].reshape(r, n)
+        jobs += [(m, Bi[j], Xi[j]) for j in range(r)]
+    with ThreadPoolExecutor(host_cpu()[0]) as ex:
+        return max(ex.map(lambda t: reldiff(t[2], orc.ldsol(t[0].a, t[0].ia, t[0].ja, t[1])), jobs))
 
 
 def bench_serving(args):
@@ -286,6 +354,8 @@ def main() -> int:
                     choices=["banded_1m", "lap2d_256", "lap3d_128", "batched", "serving"])
     ap.add_argument("--rate", type=float, default=20000.0, help="serving: offered requests per second")
     ap.add_argument("--duration", type=float, default=0.5, help="serving: seconds of arrivals")
+    ap.add_argument("--precision", choices=["both", "fp64", "fp32_refined"], default="both",
+                    help="lap3d_128 / batched: precision modes to run (profiling runs take one)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=None)
@@ -293,6 +363,10 @@ def main() -> int:
                     help="only run a plain 256 MiB device-to-device copy three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale factors)")
     args = ap.parse_args()
+    # the CPU baseline's OpenMP threads stay on their cores (read when the
+    # OpenMP runtime starts, so before anything loads it)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.copy_calibration:
         import torch
 
@@ -390,13 +464,20 @@ def main() -> int:
     launch_bytes = 12 * nnz_in + 4 * (n + 1) + 16 * n
     achieved = launch_bytes / (blk_ms * 1e-3) / 1e9
     b_alg = 12 * m.nnz + 4 * (n + 1) + 16 * n
-    traffic = None
+    # HBM traffic per block-solve launch is not measured in this run: it is
+    # read from the committed rocprofv3 counter summary (FETCH_SIZE and
+    # WRITE_SIZE runs, scaled by a plain device copy's readings), and the
+    # line names that file
+    traffic, traffic_from = None, None
     tf = Path(args.traffic_file) if args.traffic_file else ROOT / "profiles" / f"traffic_block_solve_{args.workload}.json"
     if tf.exists():
         try:
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            traffic_from = str(tf.relative_to(ROOT)) if tf.is_relative_to(ROOT) else str(tf)
         except Exception:
             traffic = None
+    solve_frac = b_alg / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    flushed_frac = b_alg / (statistics.median(fl) * 1e-3) / (HBM_PEAK_GBS * 1e9)
 
     out = {
         "metric": METRIC,
@@ -417,15 +498,16 @@ def main() -> int:
                    "jacobi" if st["reduced_sweeps"] > 0 else "exact", "reduced_sweeps": st["reduced_sweeps"]},
         "roofline": {"bound": "hbm", "kernel": "k_block_solve (pass 1 and pass 2)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": launch_bytes, "avg_launch_ms": blk_ms},
+                     "traffic_from": traffic_from, "bytes_per_launch": launch_bytes, "avg_launch_ms": blk_ms,
+                     "solve_bytes": b_alg, "solve_frac": solve_frac, "solve_frac_mall_flushed": flushed_frac},
         "phases_ms": {"pass1": ph[0], "reduced": ph[1], "coupling": ph[2], "pass2": ph[3]},
-        "solve_roofline_frac": b_alg / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "solve_roofline_frac": solve_frac,
         "latency_ms": {"p50": statistics.median(lat), "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))],
                        "mall_flushed_p50": statistics.median(fl)},
         "wall_s_timed": wall,
     }
-    # variants of the same matrix (not the metric line): 4 RHS, FP32 + refinement
-    # (FP32 results are compared with the FP64 GPU result of the same RHS)
+    # variants of the same matrix (not the metric line): 4 RHS, FP32 +
+    # refinement; every column is checked against the oracle
     var = {}
     Bv = torch.from_numpy(gen.rhs(n, 4, seed=gen.SEED_BASE + 7).reshape(4, n)).cuda()
     x64 = None
@@ -438,6 +520,7 @@ def main() -> int:
         if x64 is None:
             x64 = xh
         var[f"{prec}_nrhs{nr}"] = {"rows_rhs_per_s": n * nr / (ms_v * 1e-3), "ms": ms_v,
+                                   "rel_err_vs_oracle": oracle_err(m, B1.cpu().numpy(), xh),
                                    "rel_diff_vs_fp64_gpu": reldiff(xh, x64[:nr]),
                                    "refine_steps": fv.stats()["refine_steps"]}
         if fv is not fac:
