@@ -165,6 +165,22 @@ int spike_factor_create_partitioned(spike_factor_t *f, char uplo, int n, const d
 int spike_factor_solve(spike_factor_t f, int nrhs, const double *b, int ldb,
                        double *x, int ldx, void *stream);
 
+/* Same solve as spike_factor_solve, for callers that cannot allocate memory:
+ * code under hipGraph stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  No counterpart in the reference.  The call allocates
+ * nothing (no hipMalloc / hipFree, no staging, no graph instantiation): b and
+ * x must be device pointers (SPIKE_ERR_ARG otherwise), and the solve's
+ * workspaces (reduced vectors, coupling right-hand sides, residuals,
+ * interleaved vectors) come from a static device array of 64 MiB that every
+ * device the library runs on holds until the process ends.  A solve whose
+ * workspaces exceed it fails with SPIKE_ERR_UNSUPPORTED before launching
+ * anything.  The array is shared by all calls on one device and keeps the
+ * last call's contents (every kernel writes it before reading it): calls
+ * that overlap on one device race and must be ordered by the caller (one
+ * stream, or events).  Batch groups are not supported. */
+int spike_factor_solve_noalloc(spike_factor_t f, int nrhs, const double *b, int ldb,
+                               double *x, int ldx, void *stream);
+
 /* Same as spike_factor_solve, and adds the device time of each phase, in
  * milliseconds, to phase_ms[0..3]: pass-1 block solve, reduced solve, coupling
  * update, pass-2 block solve (hipEvents recorded on stream).  Synchronises. */

@@ -2444,6 +2444,97 @@ int spike_factor_solve_profiled(spike_factor_t f, int nrhs, const double *b, int
     return factor_solve(f, nrhs, b, ldb, x, ldx, (hipStream_t)stream, true, phase_ms);
 }
 
+int spike_factor_solve_noalloc(spike_factor_t f, int nrhs, const double *b, int ldb, double *x, int ldx,
+                               void *stream)
+{
+    if (!f) return fail(SPIKE_ERR_STATE, "factor is NULL");
+    const int n = f->plan.n;
+    if (nrhs < 0 || ldb < n || ldx < n) return fail(SPIKE_ERR_ARG, "nrhs must be >= 0 and ldb, ldx >= n");
+    if (nrhs == 0) return SPIKE_OK;
+    if (!b || !x) return fail(SPIKE_ERR_ARG, "b and x must be non-NULL");
+    if (!f->mats.empty()) return fail(SPIKE_ERR_UNSUPPORTED, "batch groups have no no-alloc solve");
+    if (int rc = take_failure(f)) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != f->device) HIP_TRY(hipSetDevice(f->device));
+    if (!is_device_ptr(b) || !is_device_ptr(x))
+        return fail(SPIKE_ERR_ARG, "the no-alloc solve takes device pointers only (host data would need staging)");
+    double *scr = nullptr;
+    HIP_TRY(spike::noalloc_scratch(&scr));
+    // Workspaces of the largest chunk, carved from the scratch: the Spike
+    // reduced vectors (y and the two Jacobi buffers), coupling right-hand
+    // sides and the FP32 residual; the level method's interleaved vectors,
+    // residual and correction.  Every kernel writes these before it reads
+    // them, so nothing needs clearing between calls.
+    const int nr0 = std::min(nrhs, std::max(f->max_nr, 1));
+    const size_t nred = f->plan.dep_rows.size(), ncpl = f->plan.cpl_rows.size();
+    size_t need = 0, slots = 1;
+    if (f->method == SPIKE_METHOD_LEVELS) {
+        if (nr0 >= 2) {
+            while ((int)slots < nr0) slots <<= 1;
+            slots = std::max<size_t>(slots, 2);
+            need = 2 * (size_t)n * slots + (f->fp32 ? (size_t)n * slots : 0);
+        } else if (f->fp32) {
+            need = 2 * (size_t)n;   // correction and residual
+        }
+    } else {
+        while ((int)slots < nr0) slots <<= 1;
+        need = (3 * nred + ncpl) * slots + (f->fp32 ? (size_t)n * slots : 0);
+    }
+    const size_t cap = spike::kNoAllocScratchBytes / sizeof(double);
+    if (need > cap) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg, "this solve needs %zu bytes of workspace; the no-alloc solve's scratch holds %zu",
+                      need * sizeof(double), spike::kNoAllocScratchBytes);
+        return fail(SPIKE_ERR_UNSUPPORTED, msg);
+    }
+    // the factor's workspace pointers point into the scratch for this call
+    struct Saved {
+        double *y, *xa, *xb, *bc, *r, *bi, *xi, *dlt;
+        size_t il_cap;
+    } sv{f->d_y, f->d_xa, f->d_xb, f->d_bc, f->d_r, f->d_bi, f->d_xi, f->d_dlt, f->il_cap};
+    double *p = scr;
+    auto take = [&](size_t count) {
+        double *q = p;
+        p += count;
+        return q;
+    };
+    if (f->method == SPIKE_METHOD_LEVELS) {
+        if (nr0 >= 2) {
+            f->d_bi = take((size_t)n * slots);
+            f->d_xi = take((size_t)n * slots);
+            f->il_cap = (size_t)n * slots;
+            if (f->fp32) f->d_r = take((size_t)n * slots);
+        } else if (f->fp32) {
+            f->d_dlt = take((size_t)n);
+            f->d_r = take((size_t)n);
+        }
+    } else {
+        f->d_y = take(nred * slots);
+        f->d_xa = take(nred * slots);
+        f->d_xb = take(nred * slots);
+        f->d_bc = take(ncpl * slots);
+        if (f->fp32) f->d_r = take((size_t)n * slots);
+    }
+    // the factor's launch sequence, chunk by chunk (no graphs, no staging)
+    int rc = SPIKE_OK;
+    for (int j = 0; j < nrhs && rc == SPIKE_OK; j += f->max_nr) {
+        const int nr = std::min(f->max_nr, nrhs - j);
+        rc = solve_chunk(f, b + (size_t)j * ldb, ldb, x + (size_t)j * ldx, ldx, nr, (hipStream_t)stream, nullptr,
+                         nullptr);
+    }
+    f->d_y = sv.y;
+    f->d_xa = sv.xa;
+    f->d_xb = sv.xb;
+    f->d_bc = sv.bc;
+    f->d_r = sv.r;
+    f->d_bi = sv.bi;
+    f->d_xi = sv.xi;
+    f->d_dlt = sv.dlt;
+    f->il_cap = sv.il_cap;
+    return rc;
+}
+
 int spike_factor_status(spike_factor_t f)
 {
     if (!f) return fail(SPIKE_ERR_ARG, "factor is NULL");

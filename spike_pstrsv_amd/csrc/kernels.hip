@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.hpp"
 
@@ -1797,6 +1798,31 @@ hipError_t launch_spike_fused(int P, const BlockArgs &a, const FusedArgs &f, int
 {
     if (P <= 0) return hipSuccess;
     return fused_dispatch(P, a, f, ncu, s, nullptr);
+}
+
+// The no-alloc entry point's scratch.  In host code the name is a stand-in,
+// not a device address: only the address from hipGetSymbolAddress (one per
+// device, each device holding its own copy) is passed to kernels.
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+hipError_t noalloc_scratch(double **ptr)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = static_cast<double *>(p);
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_fused_reset(double *X, size_t len, unsigned *ctr, int *abort_word, hipStream_t s)

@@ -235,4 +235,12 @@ hipError_t launch_copy_cols(int n, int ncols, const ColCopy &cc, hipStream_t s);
 // Largest dynamic LDS a block-solve workgroup may use, in bytes.
 size_t block_solve_lds_limit();
 
+// Device scratch of the solve entry point that allocates nothing: a
+// file-scope __device__ array of kNoAllocScratchBytes on every device the
+// library runs on (held until the process ends).  noalloc_scratch returns its
+// address on the current device, resolved with hipGetSymbolAddress once per
+// device (later calls return the kept address).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hipError_t noalloc_scratch(double **ptr);
+
 }  // namespace spike

@@ -25,7 +25,7 @@ ERRORS = {1: "SPIKE_ERR_ARG", 2: "SPIKE_ERR_MATRIX", 3: "SPIKE_ERR_DEVICE", 4: "
 EXPORTED = [
     "spike_pstrsv_preproc", "_spike_pstrsv_preproc", "spike_pstrsv", "spike_pstrsv_solve", "_spike_pstrsv_solve",
     "spike_options_default", "spike_factor_create", "spike_factor_create_partitioned", "spike_factor_solve",
-    "spike_factor_solve_profiled", "spike_factor_status",
+    "spike_factor_solve_profiled", "spike_factor_status", "spike_factor_solve_noalloc",
     "spike_factor_destroy",
     "spike_factor_stats", "spike_factor_log", "spike_analyze", "spike_pstrsv_set_partitions",
     "spike_pstrsv_current_factor", "spike_last_error", "spike_pstrsv_get_timers",
@@ -130,6 +130,8 @@ lib.spike_factor_solve.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
 lib.spike_factor_solve.restype = _i
 lib.spike_factor_solve_profiled.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp, C.POINTER(_d)]
 lib.spike_factor_solve_profiled.restype = _i
+lib.spike_factor_solve_noalloc.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
+lib.spike_factor_solve_noalloc.restype = _i
 lib.spike_factor_status.argtypes = [_vp]
 lib.spike_factor_status.restype = _i
 lib.spike_factor_destroy.argtypes = [_vp]
@@ -278,6 +280,13 @@ class Factor:
             x = np.zeros((nrhs, ldx) if nrhs > 1 else ldx, dtype=np.float64)
         _check(lib.spike_factor_solve(self.handle, nrhs, _ptr(b), ldb, _ptr(x), ldx, stream or None))
         return x
+
+    def solve_noalloc(self, b, x, nrhs: int = 1, ldb: int | None = None, ldx: int | None = None,
+                      stream: int = 0) -> None:
+        """spike_factor_solve_noalloc: device pointers only, workspaces from the library's static scratch."""
+        n = self.n
+        _check(lib.spike_factor_solve_noalloc(self.handle, nrhs, _ptr(b), n if ldb is None else ldb, _ptr(x),
+                                              n if ldx is None else ldx, stream or None))
 
     def solve_profiled(self, b, x, nrhs: int = 1, ldb: int | None = None, ldx: int | None = None,
                        stream: int = 0) -> list[float]:
