@@ -160,6 +160,7 @@ struct spike_factor_s {
     // (prepare_fused); fused_fit caches per slot class and tile whether the
     // launch fits the device (-1 unknown).
     bool fused = false;
+    bool spike_ring = false;   // spikes built by the LDS ring kernel (else direct)
     int *d_chunks = nullptr;
     int nchunks = 0;
     double *d_fx[4] = {};
@@ -720,18 +721,15 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         }
     }
 
-    // ---- spikes on the device
+    // ---- spikes on the device (layout: plan_spike_layout)
+    const spike::SpikeLayout sl = spike::plan_spike_layout(pl, spike::block_solve_lds_limit(), spike::spike_ring_lds,
+                                                           spike::spike_ring_stage_entries);
     std::vector<spike::SpikePart> sp;
-    std::vector<int64_t> woff(P, -1);
-    int64_t area = 0;
     for (int t = 0; t < P; t++) {
         const spike::Part &pt = pl.parts[t];
-        if (pt.kcnt > 0 && pt.rows_S > 0) {
-            woff[t] = area;
-            sp.push_back(spike::SpikePart{pt.s, pt.f, pt.rows_S, pt.kbeg, pt.kcnt, area});
-            area += (int64_t)pt.rows_S * pt.kcnt;
-        }
+        if (sl.slot0[t] >= 0) sp.push_back(spike::SpikePart{pt.s, pt.f, pt.rows_S, pt.kbeg, pt.kcnt, sl.slot0[t], sl.woff[t]});
     }
+    f->spike_ring = sl.ring;
     std::vector<int> hrp(nred + 1, 0);
     std::vector<int> hci;
     std::vector<double> hv;
@@ -740,11 +738,22 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     double t2 = now_s(), t3 = t2;
     if (nred > 0) {
         spike::SpikePart *d_sp;
-        int *d_kcols;
-        double *d_W;
-        if ((rc = f->upload(&d_sp, sp)) || (rc = f->upload(&d_kcols, pl.kcols)) || (rc = f->dalloc(&d_W, (size_t)area)))
+        int *d_kcols, *d_rinfo, *d_pia = nullptr, *d_pcode = nullptr;
+        double *d_W, *d_pval = nullptr;
+        const size_t npad = sl.ring ? (size_t)sl.pia.back() : 0;
+        if ((rc = f->upload(&d_sp, sp)) || (rc = f->upload(&d_kcols, pl.kcols)) ||
+            (rc = f->upload(&d_rinfo, sl.rinfo)) || (rc = f->dalloc(&d_W, (size_t)std::max<int64_t>(sl.area, 1))))
             return rc;
-        HIP_TRY(spike::launch_spike_build((int)sp.size(), d_sp, d_kcols, d_ia, d_ja, d_a, d_W, nullptr));
+        if (sl.ring) {
+            if ((rc = f->upload(&d_pia, sl.pia)) || (rc = f->dalloc(&d_pcode, npad)) || (rc = f->dalloc(&d_pval, npad)))
+                return rc;
+            HIP_TRY(spike::launch_spike_codes((int)sp.size(), sl.max_rows, d_sp, d_kcols, d_ia, d_ja, d_a, d_rinfo, d_pia,
+                                              sl.R, sl.kstride, d_pcode, d_pval, nullptr));
+            HIP_TRY(spike::launch_spike_ring((int)sp.size(), sl.max_kcnt, sl.block_threads, d_sp, d_pia, d_pcode, d_pval,
+                                             d_rinfo, sl.R, sl.nlr, sl.kstride, sl.sb, sl.cap, d_W, nullptr));
+        } else {
+            HIP_TRY(spike::launch_spike_direct((int)sp.size(), d_sp, d_kcols, d_ia, d_ja, d_a, d_W, nullptr));
+        }
         HIP_TRY(hipDeviceSynchronize());
         t2 = now_s();
         g_timers[2] += t2 - t1;
@@ -760,8 +769,8 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
             const spike::Part &pt = pl.parts[t];
             for (int q = pt.dbeg; q < pt.dbeg + pt.dcnt; q++) {
                 int r = pl.dep_rows[q] - pt.s;
-                if (woff[t] >= 0 && r >= pt.f && r <= pt.g) {
-                    wrow[q] = woff[t] + (int64_t)(r - pt.f) * pt.kcnt;
+                if (sl.woff[t] >= 0 && r >= pt.f && r <= pt.g) {
+                    wrow[q] = sl.woff[t] + (int64_t)sl.rinfo[4 * ((size_t)sl.slot0[t] + r - pt.f)] * pt.kcnt;
                     qk[q] = pt.kcnt;
                     qkb[q] = pt.kbeg;
                     tol[q] = dt > 0 ? dt / pt.kcnt : -1.0;
@@ -798,7 +807,15 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
             HIP_TRY(hipMemcpy(hv.data(), f->d_rv, sizeof(double) * rnnz, hipMemcpyDeviceToHost));
         }
         HIP_TRY(hipDeviceSynchronize());
-        dfree(f, d_W, sizeof(double) * std::max<int64_t>(area, 1));
+        dfree(f, d_W, sizeof(double) * std::max<int64_t>(sl.area, 1));
+        if (d_pcode) {
+            dfree(f, d_pval, sizeof(double) * npad);
+            dfree(f, d_pcode, sizeof(int) * npad);
+            dfree(f, d_pia, sizeof(int) * sl.pia.size());
+        }
+        dfree(f, d_rinfo, sizeof(int) * std::max<size_t>(sl.rinfo.size(), 1));
+        dfree(f, d_kcols, sizeof(int) * std::max<size_t>(pl.kcols.size(), 1));
+        dfree(f, d_sp, sizeof(spike::SpikePart) * std::max<size_t>(sp.size(), 1));
         if (!f->fp32) {
             dfree(f, d_a, sizeof(double) * pl.a.size());
             dfree(f, d_ja, sizeof(int) * pl.ja.size());

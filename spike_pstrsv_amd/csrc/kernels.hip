@@ -1597,40 +1597,270 @@ __global__ __launch_bounds__(256) void k_copy_cols(int n, ColCopy cc)
 }
 
 // ---------------------------------------------------------------- spikes
+//
+// Spike rows r = 0 .. rows-1 of a partition are its rows f .. g (row r is
+// matrix row base + r, base = s + f).  Column k of the spike (K_p's k-th
+// coupling column) satisfies, row by row,
+//     w_r = (c_r - sum_{j in-block} l_rj w_(j-base)) / l_rr,
+// c_r the row's entry in that coupling column (or 0), with the row's
+// off-diagonal entries taken in entry order.  Every kernel below runs one
+// thread per spike column and the rows in order.
 
-__global__ __launch_bounds__(256) void k_spike_build(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
+// Staged entries of the ring build.  Spike row r of a partition occupies
+// [pia[r], pia[r+1]) of (pcode, pval): first its reciprocal diagonal, then
+// its rinfo.z coupling entries (pcode = -1 - k for K_p's k-th column), zero
+// padding to a multiple of four, then its other off-diagonal entries in
+// rinfo.w chunks of four (zero-padded).  Their pcode is the byte offset
+// (x * kstride * 8) of the LDS row x that holds the column's value at the
+// entry's column:
+//   x = c & (R - 1)       spike row c = ja - base with r - c < R (ring);
+//   x = R + 1 + fslot[c]  spike row c further back (its far row);
+//   x = R                 a column of the partition before f, or padding
+//                         (zero row, value 0).
+// One wavefront per spike row; K_p is ascending.
+__global__ __launch_bounds__(256) void k_spike_codes(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
                                                      const int *__restrict__ ia, const int *__restrict__ ja,
-                                                     const double *__restrict__ a, double *W)
+                                                     const double *__restrict__ a, const int4 *__restrict__ rinfo,
+                                                     const int *__restrict__ pia, int R, int kstride, int *pcode,
+                                                     double *pval)
+{
+    const SpikePart sp = sps[blockIdx.y];
+    const int r = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (r >= sp.rows) return;
+    const int base = sp.s + sp.f, rb = kstride * (int)sizeof(double), zcode = R * rb;
+    const int4 *ri = rinfo + sp.slot0;
+    const int4 inf = ri[r];
+    const int e0 = ia[base + r], e1 = ia[base + r + 1] - 1, ncp = inf.z, nin = e1 - e0 - ncp;
+    const int q = pia[sp.slot0 + r], hdr = (1 + ncp + 3) / 4 * 4;
+    for (int l = lane; l < hdr; l += 64) {
+        int c = zcode;
+        double v = 0.0;
+        if (l == 0) {
+            v = 1.0 / a[e1];
+        } else if (l <= ncp) {
+            const int j = ja[e0 + l - 1];
+            const int *kc = kcols + sp.kbeg;
+            int lo = 0, hi = sp.kcnt - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (kc[mid] < j) lo = mid + 1;
+                else hi = mid;
+            }
+            c = -1 - lo;
+            v = a[e0 + l - 1];
+        }
+        pcode[q + l] = c;
+        pval[q + l] = v;
+    }
+    for (int m = lane; m < 4 * inf.w; m += 64) {
+        int c = zcode;
+        double v = 0.0;
+        if (m < nin) {
+            const int e = e0 + ncp + m, j = ja[e];
+            if (j >= base) {
+                const int cr = j - base;
+                c = (r - cr < R ? (cr & (R - 1)) : R + 1 + ri[cr].y) * rb;
+            }
+            v = a[e];
+        }
+        pcode[q + hdr + m] = c;
+        pval[q + hdr + m] = v;
+    }
+}
+
+// One workgroup per (partition, tile of blockDim columns), one thread per
+// column.  Each thread keeps its column's last R rows (ring), a zero row and
+// the far rows (rows that a row R or more further on reads; the host assigns
+// their far slots so that live ones never share a slot) in LDS: row x of
+// column t at lrow[x * kstride + t].  A thread reads and writes only its own
+// column there, so the rows need no barrier.  The rows' staged entries are
+// the same for every thread: they are copied sb rows at a time into LDS,
+// double-buffered, with the next stage's loads issued before the current
+// stage's rows (row offsets two stages ahead), so a row reads only LDS; its
+// row info is read one row ahead.  Per chunk of four entries a thread reads
+// the codes and values (three 16-byte reads), then per entry spends an add,
+// an LDS read and an FMA into one of four partial sums (entry position mod
+// 4), which are added at the end of the row.  rinfo per spike row: (W slot or -1, far slot or
+// -1, coupling entries, chunks).
+constexpr int kRingPF = 4;   // staged entries per thread and stage
+
+__global__ __launch_bounds__(256) void k_spike_ring(const SpikePart *__restrict__ sps, const int *__restrict__ pia,
+                                                    const int *__restrict__ pcode, const double *__restrict__ pval,
+                                                    const int4 *__restrict__ rinfo, int R, int nlr, int kstride,
+                                                    int sb, int cap, double *W)
+{
+    extern __shared__ double lds_sr[];
+    const int bd = blockDim.x, tid = threadIdx.x;
+    int4 *rib = reinterpret_cast<int4 *>(lds_sr);                       // 2 x sb row info
+    double *evb = reinterpret_cast<double *>(rib + 2 * sb);             // 2 x cap values
+    int *ecb = reinterpret_cast<int *>(evb + 2 * (size_t)cap);          // 2 x cap codes
+    double *lrow = reinterpret_cast<double *>(ecb + 2 * (size_t)cap);   // nlr x kstride
+    int *rpb = reinterpret_cast<int *>(lrow + (size_t)nlr * kstride);   // 3 x (sb + 1) row offsets
+    const SpikePart sp = sps[blockIdx.x];
+    const int col = blockIdx.y * bd + tid;
+    const bool active = col < sp.kcnt;
+    const int lt = min(tid, kstride - 1);
+    const int rows = sp.rows;
+    const int *iap = pia + sp.slot0;
+    const int4 *rip = rinfo + sp.slot0;
+    double *Wc = W + sp.woff + col;
+    const char *mine = reinterpret_cast<const char *>(lrow + lt);   // + code: this thread's entry of an LDS row
+
+    lrow[(size_t)R * kstride + lt] = 0.0;
+    int prow = 0;
+    if (tid <= sb) {
+        rpb[tid] = iap[min(tid, rows)];
+        prow = iap[min(sb + tid, rows)];
+    }
+    __syncthreads();
+    int pc[kRingPF];
+    double pv[kRingPF];
+    int4 pinf = make_int4(-1, -1, 0, 0);
+    {
+        const int e0 = rpb[0], ne = rpb[sb] - e0;
+#pragma unroll
+        for (int u = 0; u < kRingPF; u++) {
+            const int i = tid + u * bd;
+            pc[u] = i < ne ? pcode[e0 + i] : 0;
+            pv[u] = i < ne ? pval[e0 + i] : 0.0;
+        }
+        if (tid < sb && tid < rows) pinf = rip[tid];
+    }
+    for (int r0 = 0, k = 0; r0 < rows; r0 += sb, k++) {
+        const int *rp = rpb + (k % 3) * (sb + 1);
+        int *rpn = rpb + ((k + 1) % 3) * (sb + 1);
+        const int *ec = ecb + (k & 1) * cap;
+        const double *ev = evb + (k & 1) * cap;
+        const int4 *rin = rib + (k & 1) * sb;
+        {
+            int *ecw = ecb + (k & 1) * cap;
+            double *evw = evb + (k & 1) * cap;
+            const int E0 = rp[0], ne = rp[sb] - E0;
+#pragma unroll
+            for (int u = 0; u < kRingPF; u++) {
+                const int i = tid + u * bd;
+                if (i < ne) {
+                    ecw[i] = pc[u];
+                    evw[i] = pv[u];
+                }
+            }
+            if (tid < sb) rib[(k & 1) * sb + tid] = pinf;
+            if (tid <= sb) rpn[tid] = prow;
+        }
+        __syncthreads();
+        // the next stage's entries and row info, the offsets of the one after
+        const int r1n = r0 + sb;
+        if (r1n < rows) {
+            const int e0n = __builtin_amdgcn_readfirstlane(rpn[0]);
+            const int nen = __builtin_amdgcn_readfirstlane(rpn[sb]) - e0n;
+#pragma unroll
+            for (int u = 0; u < kRingPF; u++) {
+                const int i = tid + u * bd;
+                if (i < nen) {
+                    pc[u] = pcode[e0n + i];
+                    pv[u] = pval[e0n + i];
+                }
+            }
+            if (tid < sb && r1n + tid < rows) pinf = rip[r1n + tid];
+            if (tid <= sb) prow = iap[min(r1n + sb + tid, rows)];
+        }
+        const int r1 = min(r0 + sb, rows);
+        int q = 0;   // the row's first staged entry
+        int4 inf = rin[0];
+        for (int r = r0; r < r1; r++) {
+            const int4 ninf = rin[min(r + 1 - r0, sb - 1)];   // the next row's (unused after the last)
+            const int ncp = __builtin_amdgcn_readfirstlane(inf.z), nch = __builtin_amdgcn_readfirstlane(inf.w);
+            const int wsl = __builtin_amdgcn_readfirstlane(inf.x), fsl = __builtin_amdgcn_readfirstlane(inf.y);
+            const double rinv = ev[q];
+            double w = 0.0;
+            const int hdr = (1 + ncp + 3) & ~3;
+            if (ncp > 0) {
+                // coupling entries: at most one per column, so order is free
+                for (int h = 0; h < hdr; h += 4) {
+                    const int4 cd = *reinterpret_cast<const int4 *>(ec + q + h);
+                    const double2 v01 = *reinterpret_cast<const double2 *>(ev + q + h);
+                    const double2 v23 = *reinterpret_cast<const double2 *>(ev + q + h + 2);
+                    w = cd.x == -1 - col ? v01.x : w;
+                    w = cd.y == -1 - col ? v01.y : w;
+                    w = cd.z == -1 - col ? v23.x : w;
+                    w = cd.w == -1 - col ? v23.y : w;
+                }
+            }
+            q += hdr;
+            // four chunks at a time (sixteen entries): their codes, then their
+            // values in the ring, then the products, in four partial sums (one
+            // per chunk position) added at the end
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int cb = 0;
+            for (; cb + 4 <= nch; cb += 4, q += 16) {
+                int4 cd[4];
+                double2 va[4], vb[4];
+                double x[4][4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    cd[c] = *reinterpret_cast<const int4 *>(ec + q + 4 * c);
+                    va[c] = *reinterpret_cast<const double2 *>(ev + q + 4 * c);
+                    vb[c] = *reinterpret_cast<const double2 *>(ev + q + 4 * c + 2);
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    x[c][0] = *reinterpret_cast<const double *>(mine + cd[c].x);
+                    x[c][1] = *reinterpret_cast<const double *>(mine + cd[c].y);
+                    x[c][2] = *reinterpret_cast<const double *>(mine + cd[c].z);
+                    x[c][3] = *reinterpret_cast<const double *>(mine + cd[c].w);
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    s0 += va[c].x * x[c][0];
+                    s1 += va[c].y * x[c][1];
+                    s2 += vb[c].x * x[c][2];
+                    s3 += vb[c].y * x[c][3];
+                }
+            }
+            for (; cb < nch; cb++, q += 4) {
+                const int4 cd = *reinterpret_cast<const int4 *>(ec + q);
+                const double2 va = *reinterpret_cast<const double2 *>(ev + q);
+                const double2 vb = *reinterpret_cast<const double2 *>(ev + q + 2);
+                s0 += va.x * *reinterpret_cast<const double *>(mine + cd.x);
+                s1 += va.y * *reinterpret_cast<const double *>(mine + cd.y);
+                s2 += vb.x * *reinterpret_cast<const double *>(mine + cd.z);
+                s3 += vb.y * *reinterpret_cast<const double *>(mine + cd.w);
+            }
+            w -= (s0 + s1) + (s2 + s3);
+            w *= rinv;
+            if (active) {
+                lrow[(size_t)(r & (R - 1)) * kstride + lt] = w;
+                if (fsl >= 0) lrow[(size_t)(R + 1 + fsl) * kstride + lt] = w;
+                if (wsl >= 0) Wc[(int64_t)wsl * sp.kcnt] = w;
+            }
+            inf = ninf;
+        }
+    }
+}
+
+// Fallback (partitions whose ring and far rows do not fit LDS): the spike
+// rows are all stored in W (row r at W[woff + r * kcnt]) and a row reads the
+// earlier rows from there.  Threads stride over the columns.
+__global__ __launch_bounds__(256) void k_spike_direct(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
+                                                      const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const double *__restrict__ a, double *W)
 {
     const SpikePart sp = sps[blockIdx.x];
+    const int base = sp.s + sp.f;
     for (int j = threadIdx.x; j < sp.kcnt; j += blockDim.x) {
         const int myk = kcols[sp.kbeg + j];
         double *Wc = W + sp.woff + j;
-        const int base = sp.s + sp.f;
         for (int r = 0; r < sp.rows; r++) {
-            const int i = base + r;
-            const int e0 = ia[i], e1 = ia[i + 1] - 1;   // e1: diagonal
+            const int e0 = ia[base + r], e1 = ia[base + r + 1] - 1;
             double w = 0.0;
             int e = e0;
             for (; e < e1 && ja[e] < sp.s; e++)
                 if (ja[e] == myk) w = a[e];
-            // in-block entries: only spike rows (>= base) are nonzero
-            for (; e + 4 <= e1; e += 4) {
-                const int c0 = ja[e] - base, c1 = ja[e + 1] - base, c2 = ja[e + 2] - base, c3 = ja[e + 3] - base;
-                const double w0 = c0 >= 0 ? Wc[(size_t)c0 * sp.kcnt] : 0.0;
-                const double w1 = c1 >= 0 ? Wc[(size_t)c1 * sp.kcnt] : 0.0;
-                const double w2 = c2 >= 0 ? Wc[(size_t)c2 * sp.kcnt] : 0.0;
-                const double w3 = c3 >= 0 ? Wc[(size_t)c3 * sp.kcnt] : 0.0;
-                w -= a[e] * w0;
-                w -= a[e + 1] * w1;
-                w -= a[e + 2] * w2;
-                w -= a[e + 3] * w3;
-            }
             for (; e < e1; e++) {
                 const int c = ja[e] - base;
-                if (c >= 0) w -= a[e] * Wc[(size_t)c * sp.kcnt];
+                if (c >= 0) w -= a[e] * Wc[(int64_t)c * sp.kcnt];
             }
-            Wc[(size_t)r * sp.kcnt] = w / a[e1];
+            Wc[(int64_t)r * sp.kcnt] = w / a[e1];
         }
     }
 }
@@ -2096,11 +2326,52 @@ hipError_t launch_copy_cols(int n, int ncols, const ColCopy &cc, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_spike_build(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,
-                              const double *a, double *W, hipStream_t s)
+size_t spike_ring_lds(int nlr, int kstride, int sb, int cap)
+{
+    return 2 * (size_t)sb * 4 * sizeof(int) + 2 * (size_t)cap * (sizeof(double) + sizeof(int)) +
+           (size_t)nlr * kstride * sizeof(double) + 3 * (size_t)(sb + 1) * sizeof(int);
+}
+
+int spike_ring_stage_entries(int block_threads) { return kRingPF * block_threads; }
+
+hipError_t launch_spike_codes(int nsp, int max_rows, const SpikePart *sp, const int *kcols, const int *ia,
+                              const int *ja, const double *a, const int *rinfo, const int *pia, int R, int kstride,
+                              int *pcode, double *pval, hipStream_t s)
+{
+    if (nsp <= 0 || max_rows <= 0) return hipSuccess;
+    if (R < 1 || (R & (R - 1)) || kstride < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_spike_codes, dim3((unsigned)((max_rows + 3) / 4), (unsigned)nsp), dim3(256), 0, s, sp, kcols,
+                       ia, ja, a, reinterpret_cast<const int4 *>(rinfo), pia, R, kstride, pcode, pval);
+    return hipGetLastError();
+}
+
+hipError_t launch_spike_ring(int nsp, int max_kcnt, int block_threads, const SpikePart *sp, const int *pia,
+                             const int *pcode, const double *pval, const int *rinfo, int R, int nlr, int kstride,
+                             int sb, int cap, double *W, hipStream_t s)
+{
+    if (nsp <= 0 || max_kcnt <= 0) return hipSuccess;
+    if (block_threads < 64 || block_threads > 256 || block_threads % 64 || R < 1 || (R & (R - 1)) || nlr <= R ||
+        kstride < 1 || kstride > block_threads || (max_kcnt > block_threads && kstride != block_threads) ||
+        (max_kcnt <= block_threads && kstride < max_kcnt) || sb < 1 || sb >= block_threads || cap < 4 || cap % 4 ||
+        cap > kRingPF * block_threads)
+        return hipErrorInvalidValue;
+    const size_t lds = spike_ring_lds(nlr, kstride, sb, cap);
+    if (lds > block_solve_lds_limit()) return hipErrorInvalidValue;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_spike_ring, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid((unsigned)nsp, (unsigned)((max_kcnt + block_threads - 1) / block_threads));
+    hipLaunchKernelGGL(k_spike_ring, grid, dim3(block_threads), lds, s, sp, pia, pcode, pval,
+                       reinterpret_cast<const int4 *>(rinfo), R, nlr, kstride, sb, cap, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_spike_direct(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,
+                               const double *a, double *W, hipStream_t s)
 {
     if (nsp <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_spike_build, dim3(nsp), dim3(256), 0, s, sp, kcols, ia, ja, a, W);
+    hipLaunchKernelGGL(k_spike_direct, dim3(nsp), dim3(256), 0, s, sp, kcols, ia, ja, a, W);
     return hipGetLastError();
 }
 

@@ -145,14 +145,34 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
 hipError_t launch_coupling(int P, const DevPart *parts, const int *cpl_rows, const int *cpl_ptr, const int *cpl_col,
                            const double *cpl_val, const double *xred, const BlockArgs &a, double *bc, hipStream_t s);
 
-// Spike build: for every partition with kcnt > 0, W_p = D0^-1 C[:, K_p] on
-// rows [f, g] (row-major, ld = kcnt) using the lower-form CSR (ia, ja, a).
+// Spike columns of one partition: spike rows r = 0 .. rows-1 are matrix rows
+// s + f + r; K_p is kcols[kbeg .. kbeg + kcnt) (ascending); slot0 indexes
+// the partition's first spike row in per-row arrays; its W rows start at woff.
 struct SpikePart {
-    int s, f, rows, kbeg, kcnt;
+    int s, f, rows, kbeg, kcnt, slot0;
     int64_t woff;
 };
-hipError_t launch_spike_build(int nsp, const SpikePart *sp, const int *kcols, const int *ia,
-                              const int *ja, const double *a, double *W, hipStream_t s);
+// Ring spike build (layout from plan_spike_layout).  rinfo holds four ints
+// per spike row (W slot or -1, far slot or -1, coupling entries, 4-entry
+// chunks); a row's spike is stored at W[woff + wslot * kcnt] when its W slot
+// is set.  launch_spike_codes writes the staged entries (pcode, pval) at the
+// offsets pia (spike rows + 1 of them) from the lower-form CSR; R (a power of
+// two): ring rows; nlr = R + 1 + far slots; kstride: LDS row stride (>= max
+// kcnt when it fits one workgroup, else block_threads); sb (< block_threads):
+// spike rows per stage; cap (a multiple of 4, at most
+// spike_ring_stage_entries(block_threads)): the most staged entries of sb
+// consecutive spike rows.
+size_t spike_ring_lds(int nlr, int kstride, int sb, int cap);
+int spike_ring_stage_entries(int block_threads);
+hipError_t launch_spike_codes(int nsp, int max_rows, const SpikePart *sp, const int *kcols, const int *ia,
+                              const int *ja, const double *a, const int *rinfo, const int *pia, int R, int kstride,
+                              int *pcode, double *pval, hipStream_t s);
+hipError_t launch_spike_ring(int nsp, int max_kcnt, int block_threads, const SpikePart *sp, const int *pia,
+                             const int *pcode, const double *pval, const int *rinfo, int R, int nlr, int kstride,
+                             int sb, int cap, double *W, hipStream_t s);
+// Direct spike build: every spike row stored, row r at W[woff + r * kcnt].
+hipError_t launch_spike_direct(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,
+                               const double *a, double *W, hipStream_t s);
 
 // Reduced-system assembly.  For reduced row q: its spike row is
 // W[wrow[q] .. wrow[q]+kcnt[q]) (wrow < 0: identity row); entry l is kept when

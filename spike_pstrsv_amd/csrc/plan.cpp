@@ -426,4 +426,161 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     return "";
 }
 
+namespace {
+
+// Far slots for one partition: last[c] is the last row that reads spike row
+// c from R or more rows back (-1: none).  Row r first reads, then writes its
+// own far row, so a slot whose row is last read at r is free for r itself.
+// Returns the number of slots; slot (if not null) receives each row's slot.
+int color_far(const int *last, int rows, int *slot)
+{
+    std::vector<int> head(rows, -1), nxt(rows, -1), mine(rows, -1), freel;
+    for (int c = 0; c < rows; c++)
+        if (last[c] >= 0) {
+            nxt[c] = head[last[c]];
+            head[last[c]] = c;
+        }
+    int F = 0;
+    for (int r = 0; r < rows; r++) {
+        for (int c = head[r]; c >= 0; c = nxt[c]) freel.push_back(mine[c]);
+        if (last[r] >= 0) {
+            if (freel.empty()) mine[r] = F++;
+            else {
+                mine[r] = freel.back();
+                freel.pop_back();
+            }
+        }
+    }
+    if (slot) std::copy(mine.begin(), mine.end(), slot);
+    return F;
+}
+
+}  // namespace
+
+SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(int, int, int, int),
+                              int (*stage_entries)(int))
+{
+    SpikeLayout L;
+    const int P = pl.P;
+    L.slot0.assign(P, -1);
+    L.woff.assign(P, -1);
+    int nrows = 0;
+    for (int t = 0; t < P; t++) {
+        const Part &pt = pl.parts[t];
+        if (pt.kcnt <= 0 || pt.rows_S <= 0) continue;
+        L.slot0[t] = nrows;
+        nrows += pt.rows_S;
+        L.nsp++;
+        L.max_kcnt = std::max(L.max_kcnt, pt.kcnt);
+        L.max_rows = std::max(L.max_rows, pt.rows_S);
+    }
+    if (L.nsp == 0) return L;
+    const int bt = std::max(64, std::min(256, (L.max_kcnt + 63) / 64 * 64));
+    L.block_threads = bt;
+    L.kstride = L.max_kcnt > bt ? bt : L.max_kcnt;
+    L.rinfo.assign(4 * (size_t)nrows, -1);
+
+    // per spike row: coupling entries (rinfo.z), 4-entry chunks of the other
+    // off-diagonal entries (rinfo.w); last far readers per ring candidate
+    constexpr int NC = 4;
+    const int cand[NC] = {16, 32, 64, 128};
+    std::vector<int> fmax(P * NC, 0);
+    parallel_for(P, [&](int t) {
+        const Part &pt = pl.parts[t];
+        if (L.slot0[t] < 0) return;
+        const int rows = pt.rows_S, base = pt.s + pt.f;
+        int *ri = L.rinfo.data() + 4 * (size_t)L.slot0[t];
+        std::vector<int> last((size_t)NC * rows, -1);
+        for (int r = 0; r < rows; r++) {
+            int ncp = 0, nin = 0;
+            for (int e = pl.ia[base + r]; e < pl.ia[base + r + 1] - 1; e++) {
+                if (pl.ja[e] < pt.s) ncp++;
+                else nin++;
+                const int c = pl.ja[e] - base;
+                if (c < 0) continue;
+                for (int q = 0; q < NC; q++)
+                    if (r - c >= cand[q]) last[(size_t)q * rows + c] = r;
+            }
+            ri[4 * r + 2] = ncp;
+            ri[4 * r + 3] = (nin + 3) / 4;
+        }
+        for (int q = 0; q < NC; q++) fmax[t * NC + q] = color_far(last.data() + (size_t)q * rows, rows, nullptr);
+    });
+    // staged row layout: [1/diagonal, coupling entries, pad to 4 | chunks]
+    L.pia.assign((size_t)nrows + 1, 0);
+    bool pia_ok = true;
+    for (int i = 0; i < nrows && pia_ok; i++) {
+        const int64_t len = (int64_t)(1 + L.rinfo[4 * (size_t)i + 2] + 3) / 4 * 4 + 4 * (int64_t)L.rinfo[4 * (size_t)i + 3];
+        const int64_t nx = L.pia[i] + len;
+        pia_ok = nx <= INT32_MAX;
+        L.pia[i + 1] = pia_ok ? (int)nx : 0;
+    }
+    int best = -1;
+    int64_t best_rows = 0;
+    for (int q = 0; q < NC; q++) {
+        int F = 0;
+        for (int t = 0; t < P; t++) F = std::max(F, fmax[t * NC + q]);
+        const int64_t nl = cand[q] + 1 + F;
+        if (best < 0 || nl < best_rows || (nl == best_rows && cand[q] > cand[best])) {
+            best = q;
+            best_rows = nl;
+        }
+    }
+    // rows per stage: the most (<= 32) whose staged entries fit a stage
+    int sb = 0, cap = 0;
+    for (int cs = 32; cs >= 1 && sb == 0 && pia_ok; cs >>= 1) {
+        int m = 0;
+        for (int t = 0; t < P; t++) {
+            if (L.slot0[t] < 0) continue;
+            const int *pr = L.pia.data() + L.slot0[t], rows = pl.parts[t].rows_S;
+            for (int r0 = 0; r0 < rows; r0 += cs) m = std::max(m, pr[std::min(r0 + cs, rows)] - pr[r0]);
+        }
+        if (m <= stage_entries(bt)) {
+            sb = cs;
+            cap = m;
+        }
+    }
+    L.ring = sb > 0 && sb < bt && best_rows <= INT32_MAX &&
+             lds((int)best_rows, L.kstride, sb, cap) <= lds_limit;
+    if (L.ring) {
+        L.R = cand[best];
+        L.nlr = (int)best_rows;
+        L.sb = sb;
+        L.cap = cap;
+        parallel_for(P, [&](int t) {
+            const Part &pt = pl.parts[t];
+            if (L.slot0[t] < 0) return;
+            const int rows = pt.rows_S, base = pt.s + pt.f;
+            std::vector<int> last(rows, -1), slot(rows, -1);
+            for (int r = 0; r < rows; r++)
+                for (int e = pl.ia[base + r]; e < pl.ia[base + r + 1] - 1; e++) {
+                    const int c = pl.ja[e] - base;
+                    if (c >= 0 && r - c >= L.R) last[c] = r;
+                }
+            color_far(last.data(), rows, slot.data());
+            int *ri = L.rinfo.data() + 4 * (size_t)L.slot0[t];
+            for (int r = 0; r < rows; r++) ri[4 * r + 1] = slot[r];
+        });
+    }
+    // W slots: the dependents (ring), every spike row (direct)
+    for (int t = 0; t < P; t++) {
+        const Part &pt = pl.parts[t];
+        if (L.slot0[t] < 0) continue;
+        int *ri = L.rinfo.data() + 4 * (size_t)L.slot0[t];
+        int kept = 0;
+        if (L.ring) {
+            for (int q = pt.dbeg; q < pt.dbeg + pt.dcnt; q++) {
+                const int r = pl.dep_rows[q] - pt.s;
+                if (r >= pt.f && r <= pt.g) ri[4 * (r - pt.f)] = 1;
+            }
+            for (int r = 0; r < pt.rows_S; r++) ri[4 * r] = ri[4 * r] > 0 ? kept++ : -1;
+        } else {
+            for (int r = 0; r < pt.rows_S; r++) ri[4 * r] = kept++;
+        }
+        L.woff[t] = L.area;
+        L.area += (int64_t)kept * pt.kcnt;
+    }
+    return L;
+}
+
 }  // namespace spike

@@ -108,6 +108,32 @@ struct PlanOptions {
 std::string build_plan(char uplo, int n, const double *a, const int *ia, const int *ja,
                        const PlanOptions &opt, Plan &plan);
 
+// Spike build layout.  Every spike row (rows f .. g of a partition with a
+// spike, numbered from slot0[t]) gets rinfo = (W slot, far slot, coupling
+// entries, 4-entry chunks of its other off-diagonal entries): the W slot
+// numbers the rows that are stored in W, the far slot the LDS row that keeps
+// the row for a row R or more further on (ring build; live far rows never
+// share a slot).  pia: offsets of the rows in the ring build's staged entry
+// layout (a row: its reciprocal diagonal, its coupling entries, padding to a
+// multiple of four, then its other entries in chunks of four).  Ring build: only the dependents are stored, R ring
+// rows (a power of two) and the far rows fit lds_limit with sb spike rows
+// per stage of at most cap entries.  Direct build (ring = false): every
+// spike row is stored (W slot = its row).
+struct SpikeLayout {
+    bool ring = false;
+    int R = 0, nlr = 0, kstride = 0, sb = 0, cap = 0, block_threads = 64;
+    int max_kcnt = 0, max_rows = 0, nsp = 0;
+    std::vector<int> slot0;     // P, -1 for a partition without a spike
+    std::vector<int> rinfo;     // 4 per spike row
+    std::vector<int> pia;       // spike rows + 1
+    std::vector<int64_t> woff;  // P, -1 for a partition without a spike
+    int64_t area = 0;           // W entries
+};
+// lds(nlr, kstride, sb, cap): LDS bytes of the ring build; stage_entries:
+// most staged entries per stage for block_threads threads.
+SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(int, int, int, int),
+                              int (*stage_entries)(int));
+
 // Lanes per row of the level kernel: the smallest power of two (<= 16) that
 // covers the 90th percentile of the row lengths.
 int choose_lanes(const std::vector<int> &row_len);
