@@ -722,8 +722,9 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     }
 
     // ---- spikes on the device (layout: plan_spike_layout)
+    const char *gfe = std::getenv("SPIKE_PSTRSV_SPIKE_GFAR");
     const spike::SpikeLayout sl = spike::plan_spike_layout(pl, spike::block_solve_lds_limit(), spike::spike_ring_lds,
-                                                           spike::spike_ring_stage_entries);
+                                                           spike::spike_ring_stage_entries, !(gfe && std::atoi(gfe) == 0));
     std::vector<spike::SpikePart> sp;
     for (int t = 0; t < P; t++) {
         const spike::Part &pt = pl.parts[t];
@@ -748,11 +749,11 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
             if ((rc = f->upload(&d_pia, sl.pia)) || (rc = f->dalloc(&d_pcode, npad)) || (rc = f->dalloc(&d_pval, npad)))
                 return rc;
             HIP_TRY(spike::launch_spike_codes((int)sp.size(), sl.max_rows, d_sp, d_kcols, d_ia, d_ja, d_a, d_rinfo, d_pia,
-                                              sl.R, sl.kstride, d_pcode, d_pval, nullptr));
+                                              sl.R, sl.kstride, sl.gfar, d_pcode, d_pval, nullptr));
             HIP_TRY(spike::launch_spike_ring((int)sp.size(), sl.max_kcnt, sl.block_threads, d_sp, d_pia, d_pcode, d_pval,
-                                             d_rinfo, sl.R, sl.nlr, sl.kstride, sl.sb, sl.cap, d_W, nullptr));
+                                             d_rinfo, sl.R, sl.nlr, sl.kstride, sl.sb, sl.cap, sl.gfar, d_W, nullptr));
         } else {
-            HIP_TRY(spike::launch_spike_direct((int)sp.size(), d_sp, d_kcols, d_ia, d_ja, d_a, d_W, nullptr));
+            HIP_TRY(spike::launch_spike_direct((int)sp.size(), sl.max_kcnt, d_sp, d_kcols, d_ia, d_ja, d_a, d_W, nullptr));
         }
         HIP_TRY(hipDeviceSynchronize());
         t2 = now_s();

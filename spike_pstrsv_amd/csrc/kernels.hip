@@ -1617,12 +1617,13 @@ __global__ __launch_bounds__(256) void k_copy_cols(int n, ColCopy cc)
 //   x = R + 1 + fslot[c]  spike row c further back (its far row);
 //   x = R                 a column of the partition before f, or padding
 //                         (zero row, value 0).
-// One wavefront per spike row; K_p is ascending.
+// With gfar, a spike row c further back is read from W instead: pcode =
+// -1 - wslot[c].  One wavefront per spike row; K_p is ascending.
 __global__ __launch_bounds__(256) void k_spike_codes(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
                                                      const int *__restrict__ ia, const int *__restrict__ ja,
                                                      const double *__restrict__ a, const int4 *__restrict__ rinfo,
-                                                     const int *__restrict__ pia, int R, int kstride, int *pcode,
-                                                     double *pval)
+                                                     const int *__restrict__ pia, int R, int kstride, int gfar,
+                                                     int *pcode, double *pval)
 {
     const SpikePart sp = sps[blockIdx.y];
     const int r = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -1659,7 +1660,7 @@ __global__ __launch_bounds__(256) void k_spike_codes(const SpikePart *__restrict
             const int e = e0 + ncp + m, j = ja[e];
             if (j >= base) {
                 const int cr = j - base;
-                c = (r - cr < R ? (cr & (R - 1)) : R + 1 + ri[cr].y) * rb;
+                c = r - cr < R ? (cr & (R - 1)) * rb : gfar ? -1 - ri[cr].x : (R + 1 + ri[cr].y) * rb;
             }
             v = a[e];
         }
@@ -1684,6 +1685,17 @@ __global__ __launch_bounds__(256) void k_spike_codes(const SpikePart *__restrict
 // -1, coupling entries, chunks).
 constexpr int kRingPF = 4;   // staged entries per thread and stage
 
+// Value of this thread's column at a chunk entry's code: an LDS row, or
+// (GF, negative code) a row stored in W (a flat load covers both).
+template <bool GF>
+__device__ __forceinline__ double ring_value(const char *mine, const double *Wr, int kcnt, int code)
+{
+    if (!GF) return *reinterpret_cast<const double *>(mine + code);
+    const double *p = code >= 0 ? reinterpret_cast<const double *>(mine + code) : Wr + (int64_t)(-1 - code) * kcnt;
+    return *p;
+}
+
+template <bool GF>
 __global__ __launch_bounds__(256) void k_spike_ring(const SpikePart *__restrict__ sps, const int *__restrict__ pia,
                                                     const int *__restrict__ pcode, const double *__restrict__ pval,
                                                     const int4 *__restrict__ rinfo, int R, int nlr, int kstride,
@@ -1704,6 +1716,7 @@ __global__ __launch_bounds__(256) void k_spike_ring(const SpikePart *__restrict_
     const int *iap = pia + sp.slot0;
     const int4 *rip = rinfo + sp.slot0;
     double *Wc = W + sp.woff + col;
+    const double *Wr = W + sp.woff + min(col, sp.kcnt - 1);          // far reads (idle lanes: a real column)
     const char *mine = reinterpret_cast<const char *>(lrow + lt);   // + code: this thread's entry of an LDS row
 
     lrow[(size_t)R * kstride + lt] = 0.0;
@@ -1804,10 +1817,10 @@ __global__ __launch_bounds__(256) void k_spike_ring(const SpikePart *__restrict_
                 }
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    x[c][0] = *reinterpret_cast<const double *>(mine + cd[c].x);
-                    x[c][1] = *reinterpret_cast<const double *>(mine + cd[c].y);
-                    x[c][2] = *reinterpret_cast<const double *>(mine + cd[c].z);
-                    x[c][3] = *reinterpret_cast<const double *>(mine + cd[c].w);
+                    x[c][0] = ring_value<GF>(mine, Wr, sp.kcnt, cd[c].x);
+                    x[c][1] = ring_value<GF>(mine, Wr, sp.kcnt, cd[c].y);
+                    x[c][2] = ring_value<GF>(mine, Wr, sp.kcnt, cd[c].z);
+                    x[c][3] = ring_value<GF>(mine, Wr, sp.kcnt, cd[c].w);
                 }
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
@@ -1821,10 +1834,10 @@ __global__ __launch_bounds__(256) void k_spike_ring(const SpikePart *__restrict_
                 const int4 cd = *reinterpret_cast<const int4 *>(ec + q);
                 const double2 va = *reinterpret_cast<const double2 *>(ev + q);
                 const double2 vb = *reinterpret_cast<const double2 *>(ev + q + 2);
-                s0 += va.x * *reinterpret_cast<const double *>(mine + cd.x);
-                s1 += va.y * *reinterpret_cast<const double *>(mine + cd.y);
-                s2 += vb.x * *reinterpret_cast<const double *>(mine + cd.z);
-                s3 += vb.y * *reinterpret_cast<const double *>(mine + cd.w);
+                s0 += va.x * ring_value<GF>(mine, Wr, sp.kcnt, cd.x);
+                s1 += va.y * ring_value<GF>(mine, Wr, sp.kcnt, cd.y);
+                s2 += vb.x * ring_value<GF>(mine, Wr, sp.kcnt, cd.z);
+                s3 += vb.y * ring_value<GF>(mine, Wr, sp.kcnt, cd.w);
             }
             w -= (s0 + s1) + (s2 + s3);
             w *= rinv;
@@ -1838,30 +1851,31 @@ __global__ __launch_bounds__(256) void k_spike_ring(const SpikePart *__restrict_
     }
 }
 
-// Fallback (partitions whose ring and far rows do not fit LDS): the spike
-// rows are all stored in W (row r at W[woff + r * kcnt]) and a row reads the
-// earlier rows from there.  Threads stride over the columns.
+// Fallback (partitions whose ring does not fit LDS): the spike rows are all
+// stored in W (row r at W[woff + r * kcnt]) and a row reads the earlier rows
+// from there.  One thread per column: workgroup (p, y) takes columns
+// [256 y, 256 y + 256) of partition p.
 __global__ __launch_bounds__(256) void k_spike_direct(const SpikePart *__restrict__ sps, const int *__restrict__ kcols,
                                                       const int *__restrict__ ia, const int *__restrict__ ja,
                                                       const double *__restrict__ a, double *W)
 {
     const SpikePart sp = sps[blockIdx.x];
     const int base = sp.s + sp.f;
-    for (int j = threadIdx.x; j < sp.kcnt; j += blockDim.x) {
-        const int myk = kcols[sp.kbeg + j];
-        double *Wc = W + sp.woff + j;
-        for (int r = 0; r < sp.rows; r++) {
-            const int e0 = ia[base + r], e1 = ia[base + r + 1] - 1;
-            double w = 0.0;
-            int e = e0;
-            for (; e < e1 && ja[e] < sp.s; e++)
-                if (ja[e] == myk) w = a[e];
-            for (; e < e1; e++) {
-                const int c = ja[e] - base;
-                if (c >= 0) w -= a[e] * Wc[(int64_t)c * sp.kcnt];
-            }
-            Wc[(int64_t)r * sp.kcnt] = w / a[e1];
+    const int j = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+    if (j >= sp.kcnt) return;
+    const int myk = kcols[sp.kbeg + j];
+    double *Wc = W + sp.woff + j;
+    for (int r = 0; r < sp.rows; r++) {
+        const int e0 = ia[base + r], e1 = ia[base + r + 1] - 1;
+        double w = 0.0;
+        int e = e0;
+        for (; e < e1 && ja[e] < sp.s; e++)
+            if (ja[e] == myk) w = a[e];
+        for (; e < e1; e++) {
+            const int c = ja[e] - base;
+            if (c >= 0) w -= a[e] * Wc[(int64_t)c * sp.kcnt];
         }
+        Wc[(int64_t)r * sp.kcnt] = w / a[e1];
     }
 }
 
@@ -2336,18 +2350,18 @@ int spike_ring_stage_entries(int block_threads) { return kRingPF * block_threads
 
 hipError_t launch_spike_codes(int nsp, int max_rows, const SpikePart *sp, const int *kcols, const int *ia,
                               const int *ja, const double *a, const int *rinfo, const int *pia, int R, int kstride,
-                              int *pcode, double *pval, hipStream_t s)
+                              bool gfar, int *pcode, double *pval, hipStream_t s)
 {
     if (nsp <= 0 || max_rows <= 0) return hipSuccess;
     if (R < 1 || (R & (R - 1)) || kstride < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_spike_codes, dim3((unsigned)((max_rows + 3) / 4), (unsigned)nsp), dim3(256), 0, s, sp, kcols,
-                       ia, ja, a, reinterpret_cast<const int4 *>(rinfo), pia, R, kstride, pcode, pval);
+                       ia, ja, a, reinterpret_cast<const int4 *>(rinfo), pia, R, kstride, gfar ? 1 : 0, pcode, pval);
     return hipGetLastError();
 }
 
 hipError_t launch_spike_ring(int nsp, int max_kcnt, int block_threads, const SpikePart *sp, const int *pia,
                              const int *pcode, const double *pval, const int *rinfo, int R, int nlr, int kstride,
-                             int sb, int cap, double *W, hipStream_t s)
+                             int sb, int cap, bool gfar, double *W, hipStream_t s)
 {
     if (nsp <= 0 || max_kcnt <= 0) return hipSuccess;
     if (block_threads < 64 || block_threads > 256 || block_threads % 64 || R < 1 || (R & (R - 1)) || nlr <= R ||
@@ -2357,21 +2371,27 @@ hipError_t launch_spike_ring(int nsp, int max_kcnt, int block_threads, const Spi
         return hipErrorInvalidValue;
     const size_t lds = spike_ring_lds(nlr, kstride, sb, cap);
     if (lds > block_solve_lds_limit()) return hipErrorInvalidValue;
+    const void *fn = gfar ? (const void *)k_spike_ring<true> : (const void *)k_spike_ring<false>;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_spike_ring, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const dim3 grid((unsigned)nsp, (unsigned)((max_kcnt + block_threads - 1) / block_threads));
-    hipLaunchKernelGGL(k_spike_ring, grid, dim3(block_threads), lds, s, sp, pia, pcode, pval,
-                       reinterpret_cast<const int4 *>(rinfo), R, nlr, kstride, sb, cap, W);
+    if (gfar)
+        hipLaunchKernelGGL(k_spike_ring<true>, grid, dim3(block_threads), lds, s, sp, pia, pcode, pval,
+                           reinterpret_cast<const int4 *>(rinfo), R, nlr, kstride, sb, cap, W);
+    else
+        hipLaunchKernelGGL(k_spike_ring<false>, grid, dim3(block_threads), lds, s, sp, pia, pcode, pval,
+                           reinterpret_cast<const int4 *>(rinfo), R, nlr, kstride, sb, cap, W);
     return hipGetLastError();
 }
 
-hipError_t launch_spike_direct(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,
-                               const double *a, double *W, hipStream_t s)
+hipError_t launch_spike_direct(int nsp, int max_kcnt, const SpikePart *sp, const int *kcols, const int *ia,
+                               const int *ja, const double *a, double *W, hipStream_t s)
 {
-    if (nsp <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_spike_direct, dim3(nsp), dim3(256), 0, s, sp, kcols, ia, ja, a, W);
+    if (nsp <= 0 || max_kcnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spike_direct, dim3((unsigned)nsp, (unsigned)((max_kcnt + 255) / 256)), dim3(256), 0, s, sp,
+                       kcols, ia, ja, a, W);
     return hipGetLastError();
 }
 

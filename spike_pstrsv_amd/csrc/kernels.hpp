@@ -161,18 +161,19 @@ struct SpikePart {
 // kcnt when it fits one workgroup, else block_threads); sb (< block_threads):
 // spike rows per stage; cap (a multiple of 4, at most
 // spike_ring_stage_entries(block_threads)): the most staged entries of sb
-// consecutive spike rows.
+// consecutive spike rows.  gfar: far rows are read from W (their W slots
+// set), nlr = R + 1.
 size_t spike_ring_lds(int nlr, int kstride, int sb, int cap);
 int spike_ring_stage_entries(int block_threads);
 hipError_t launch_spike_codes(int nsp, int max_rows, const SpikePart *sp, const int *kcols, const int *ia,
                               const int *ja, const double *a, const int *rinfo, const int *pia, int R, int kstride,
-                              int *pcode, double *pval, hipStream_t s);
+                              bool gfar, int *pcode, double *pval, hipStream_t s);
 hipError_t launch_spike_ring(int nsp, int max_kcnt, int block_threads, const SpikePart *sp, const int *pia,
                              const int *pcode, const double *pval, const int *rinfo, int R, int nlr, int kstride,
-                             int sb, int cap, double *W, hipStream_t s);
+                             int sb, int cap, bool gfar, double *W, hipStream_t s);
 // Direct spike build: every spike row stored, row r at W[woff + r * kcnt].
-hipError_t launch_spike_direct(int nsp, const SpikePart *sp, const int *kcols, const int *ia, const int *ja,
-                               const double *a, double *W, hipStream_t s);
+hipError_t launch_spike_direct(int nsp, int max_kcnt, const SpikePart *sp, const int *kcols, const int *ia,
+                               const int *ja, const double *a, double *W, hipStream_t s);
 
 // Reduced-system assembly.  For reduced row q: its spike row is
 // W[wrow[q] .. wrow[q]+kcnt[q]) (wrow < 0: identity row); entry l is kept when

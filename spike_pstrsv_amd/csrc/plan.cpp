@@ -458,7 +458,7 @@ int color_far(const int *last, int rows, int *slot)
 }  // namespace
 
 SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(int, int, int, int),
-                              int (*stage_entries)(int))
+                              int (*stage_entries)(int), bool allow_gfar)
 {
     SpikeLayout L;
     const int P = pl.P;
@@ -485,6 +485,7 @@ SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(in
     constexpr int NC = 4;
     const int cand[NC] = {16, 32, 64, 128};
     std::vector<int> fmax(P * NC, 0);
+    std::vector<int64_t> nfar(P * NC, 0);   // entries read from R or more rows back
     parallel_for(P, [&](int t) {
         const Part &pt = pl.parts[t];
         if (L.slot0[t] < 0) return;
@@ -499,7 +500,10 @@ SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(in
                 const int c = pl.ja[e] - base;
                 if (c < 0) continue;
                 for (int q = 0; q < NC; q++)
-                    if (r - c >= cand[q]) last[(size_t)q * rows + c] = r;
+                    if (r - c >= cand[q]) {
+                        last[(size_t)q * rows + c] = r;
+                        nfar[t * NC + q]++;
+                    }
             }
             ri[4 * r + 2] = ncp;
             ri[4 * r + 3] = (nin + 3) / 4;
@@ -540,8 +544,26 @@ SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(in
             cap = m;
         }
     }
-    L.ring = sb > 0 && sb < bt && best_rows <= INT32_MAX &&
-             lds((int)best_rows, L.kstride, sb, cap) <= lds_limit;
+    const bool stage_ok = sb > 0 && sb < bt;
+    L.ring = stage_ok && best_rows <= INT32_MAX && lds((int)best_rows, L.kstride, sb, cap) <= lds_limit;
+    if (!L.ring && stage_ok && allow_gfar) {
+        // far rows in W instead of LDS: the largest ring that fits, when it
+        // leaves at most two far entries per row (each a memory round trip
+        // on the row's chain; with more, the direct build is faster: a
+        // 4096-matrix batch of bandwidths up to 128 built in 274 ms of
+        // kernel time this way against 189 direct, while 2D Laplacians with
+        // one far entry per row took 136 ms against 204 for 64 serving
+        // matrices)
+        for (int q = NC - 1; q >= 0 && !L.ring; q--) {
+            int64_t far = 0;
+            for (int t = 0; t < P; t++) far += nfar[t * NC + q];
+            if (far <= 2LL * nrows && lds(cand[q] + 1, L.kstride, sb, cap) <= lds_limit) {
+                L.ring = L.gfar = true;
+                best = q;
+                best_rows = cand[q] + 1;
+            }
+        }
+    }
     if (L.ring) {
         L.R = cand[best];
         L.nlr = (int)best_rows;
@@ -557,12 +579,18 @@ SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(in
                     const int c = pl.ja[e] - base;
                     if (c >= 0 && r - c >= L.R) last[c] = r;
                 }
-            color_far(last.data(), rows, slot.data());
             int *ri = L.rinfo.data() + 4 * (size_t)L.slot0[t];
+            if (L.gfar) {
+                // far-read rows get W slots (numbered below with the dependents)
+                for (int r = 0; r < rows; r++) ri[4 * r] = last[r] >= 0 ? 1 : -1;
+                return;
+            }
+            color_far(last.data(), rows, slot.data());
             for (int r = 0; r < rows; r++) ri[4 * r + 1] = slot[r];
         });
     }
-    // W slots: the dependents (ring), every spike row (direct)
+    // W slots: the dependents (and, far rows in W, the far-read rows) for the
+    // ring build, every spike row for the direct build
     for (int t = 0; t < P; t++) {
         const Part &pt = pl.parts[t];
         if (L.slot0[t] < 0) continue;

@@ -113,14 +113,15 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
 // entries, 4-entry chunks of its other off-diagonal entries): the W slot
 // numbers the rows that are stored in W, the far slot the LDS row that keeps
 // the row for a row R or more further on (ring build; live far rows never
-// share a slot).  pia: offsets of the rows in the ring build's staged entry
+// share a slot; when those LDS rows do not fit, gfar: the far-read rows are
+// stored in W and read from there, far slot -1).  pia: offsets of the rows in the ring build's staged entry
 // layout (a row: its reciprocal diagonal, its coupling entries, padding to a
 // multiple of four, then its other entries in chunks of four).  Ring build: only the dependents are stored, R ring
 // rows (a power of two) and the far rows fit lds_limit with sb spike rows
 // per stage of at most cap entries.  Direct build (ring = false): every
 // spike row is stored (W slot = its row).
 struct SpikeLayout {
-    bool ring = false;
+    bool ring = false, gfar = false;   // gfar: far rows read from W, not LDS
     int R = 0, nlr = 0, kstride = 0, sb = 0, cap = 0, block_threads = 64;
     int max_kcnt = 0, max_rows = 0, nsp = 0;
     std::vector<int> slot0;     // P, -1 for a partition without a spike
@@ -132,7 +133,7 @@ struct SpikeLayout {
 // lds(nlr, kstride, sb, cap): LDS bytes of the ring build; stage_entries:
 // most staged entries per stage for block_threads threads.
 SpikeLayout plan_spike_layout(const Plan &pl, size_t lds_limit, size_t (*lds)(int, int, int, int),
-                              int (*stage_entries)(int));
+                              int (*stage_entries)(int), bool allow_gfar);
 
 // Lanes per row of the level kernel: the smallest power of two (<= 16) that
 // covers the 90th percentile of the row lengths.
