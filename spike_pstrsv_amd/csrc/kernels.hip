@@ -1172,6 +1172,83 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(int nrows, const int4 *__r
         for (int t = 0; t < NR; t++) xout[(size_t)r * NR + t] = base[t] - acc[t];
 }
 
+// One Jacobi sweep m over items (runs of one partition's dependents; see
+// JacobiItems).  A workgroup first gathers xin at its item's distinct columns
+// into LDS, NR values per column (only when the list code has the top bit
+// set: some row of the item computes in this sweep), then walks its rows
+// with 32 groups of 8 lanes: rows with need >= m compute xout = y - W xin
+// (lane li adds entries li, li+8, ... in order, four per trip, then
+// group_sum<8>: the order of k_jacobi_sweep<8, 4, NR>, so both give the same
+// bits), rows with need == m-1 (or need == 0 at m == 2) copy xin to xout,
+// and the others (already final in both buffers) are left alone.
+template <int NR>
+__global__ __launch_bounds__(256) void k_jacobi_items(JacobiItems J, int m, const int *__restrict__ list,
+                                                      const double *__restrict__ xin, double *__restrict__ xout)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_j[];
+    double *xk = reinterpret_cast<double *>(smem_j);
+    constexpr int UG = 4;
+    const int tid = threadIdx.x, g = tid >> 3, li = tid & 7;
+    const int code = list[blockIdx.x];
+    const int4 it = reinterpret_cast<const int4 *>(J.items)[code & 0x7fffffff];
+    const int q0 = it.x, q1 = it.y, kb = it.z, nkv = (it.w - it.z) * NR;
+    if (code < 0) {
+        for (int i0 = 0; i0 < nkv; i0 += 256 * UG) {
+            double xv[UG];
+#pragma unroll
+            for (int u = 0; u < UG; u++) {
+                const int i = min(i0 + tid + 256 * u, nkv - 1);
+                xv[u] = xin[(size_t)J.kcol[kb + i / NR] * NR + (unsigned)(i % NR)];
+            }
+#pragma unroll
+            for (int u = 0; u < UG; u++) {
+                const int i = i0 + tid + 256 * u;
+                if (i < nkv) xk[i] = xv[u];
+            }
+        }
+        __syncthreads();
+    }
+    for (int q = q0 + g; q < q1; q += 32) {
+        const int nd = J.need[q];
+        if (nd >= m) {
+            const int e1 = J.rp[q + 1];
+            double acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] = 0.0;
+            for (int e = J.rp[q] + li; e < e1; e += 32) {
+                int c[4];
+                double w[4], xv[4][NR];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool ok = e + 8 * u < e1;
+                    const int ee = ok ? e + 8 * u : e1 - 1;
+                    const int cv = J.lci[ee];
+                    const double wv = J.v[ee];
+                    c[u] = ok ? cv : 0;
+                    w[u] = ok ? wv : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < NR; r++) xv[u][r] = xk[c[u] * NR + r];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (e + 8 * u < e1)
+#pragma unroll
+                        for (int r = 0; r < NR; r++) acc[r] = fma(w[u], xv[u][r], acc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] = group_sum<8>(acc[r]);
+            if (li == 0)
+#pragma unroll
+                for (int r = 0; r < NR; r++) xout[(size_t)q * NR + r] = J.y[(size_t)q * NR + r] - acc[r];
+        } else if (nd == m - 1 || (nd == 0 && m == 2)) {
+            if (li < NR)
+                for (int r = li; r < NR; r += 8) xout[(size_t)q * NR + r] = xin[(size_t)q * NR + r];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- coupling
 
 // One workgroup per partition: threads over its coupling rows.
@@ -2130,6 +2207,39 @@ hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const
     case 4: jacobi_go<8, 4, 4>(nrows, r4, ci, v, y, xin, xout, s); break;
     case 8: jacobi_go<8, 4, 8>(nrows, r4, ci, v, y, xin, xout, s); break;
     default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t jacobi_items_lds(int kmax, int nr_slots)
+{
+    return (size_t)std::max(kmax, 1) * (size_t)nr_slots * sizeof(double);
+}
+
+hipError_t launch_jacobi_items(const JacobiItems &j, int m, const int *list, int nitems, const double *xin,
+                               double *xout, int nr_slots, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    const size_t lds = jacobi_items_lds(j.kmax, nr_slots);
+    if (lds > block_solve_lds_limit()) return hipErrorInvalidValue;
+    const void *fn = nullptr;
+    switch (nr_slots) {
+    case 1: fn = (const void *)k_jacobi_items<1>; break;
+    case 2: fn = (const void *)k_jacobi_items<2>; break;
+    case 4: fn = (const void *)k_jacobi_items<4>; break;
+    case 8: fn = (const void *)k_jacobi_items<8>; break;
+    default: return hipErrorInvalidValue;
+    }
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid((unsigned)nitems), block(256);
+    switch (nr_slots) {
+    case 1: hipLaunchKernelGGL(k_jacobi_items<1>, grid, block, lds, s, j, m, list, xin, xout); break;
+    case 2: hipLaunchKernelGGL(k_jacobi_items<2>, grid, block, lds, s, j, m, list, xin, xout); break;
+    case 4: hipLaunchKernelGGL(k_jacobi_items<4>, grid, block, lds, s, j, m, list, xin, xout); break;
+    case 8: hipLaunchKernelGGL(k_jacobi_items<8>, grid, block, lds, s, j, m, list, xin, xout); break;
     }
     return hipGetLastError();
 }

@@ -136,6 +136,27 @@ hipError_t launch_reduced_solve(int nred, int nlev, const int *levptr, const int
 hipError_t launch_jacobi_sweep(int nrows, const int *rows4, const int *ci, const double *v, const double *y,
                                const double *xin, double *xout, int nr_slots, hipStream_t s);
 
+// Jacobi sweeps over items: runs of consecutive dependents of one partition
+// (reduced rows q0 .. q1).  An item's W entries (reduced CSR rp / v) read
+// only the columns kcol[kb .. ke) (reduced indices, ascending), addressed by
+// their position lci; row q runs need[q] sweeps.  Sweep m launches the items
+// of list (top bit: some row of the item computes, so the item gathers xin
+// at its columns into LDS once).  Same results, bit for bit, as
+// launch_jacobi_sweep over the same sweep's row list.
+struct JacobiItems {
+    const int *items;       // int4 per item: q0, q1, kb, ke
+    const int *kcol;
+    const int *rp;
+    const uint16_t *lci;
+    const double *v;
+    const int *need;
+    const double *y;
+    int kmax;               // most columns of an item
+};
+size_t jacobi_items_lds(int kmax, int nr_slots);
+hipError_t launch_jacobi_items(const JacobiItems &j, int m, const int *list, int nitems, const double *xin,
+                               double *xout, int nr_slots, hipStream_t s);
+
 // bc[c] = b[row_c] - sum_j val_j * xred[col_j] for every coupling row c and
 // right-hand side (bc, xred interleaved with a.nr_slots values per index); b
 // is addressed per partition as in the block solve (a.b, a.ldb, a.nr).  The

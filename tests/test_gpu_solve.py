@@ -337,3 +337,23 @@ def test_partition_log_parses_like_the_reference_tool(native, lap2d, capfd):
     for p in per:
         r, c = (int(v) for v in p["S"].split(" x "))
         assert p["hasToCalculateS"] == (r > 0 and c > 0)
+
+
+@pytest.mark.parametrize("nrhs,precision", [(1, "fp64"), (3, "fp64"), (8, "fp64"), (4, "fp32_refined")])
+def test_jacobi_items_match_row_sweeps(native, monkeypatch, nrhs, precision):
+    """Jacobi sweeps over items (one gather per distinct column into LDS) give
+    the bits of the row-list sweeps, at every RHS count."""
+    m = gen.banded(1 << 18, seed=gen.SEED_BASE + 43)
+    B = gen.rhs(m.n, nrhs, seed=gen.SEED_BASE + 44)
+    monkeypatch.setenv("SPIKE_PSTRSV_JACOBI_ITEMS", "0")
+    f0 = native.Factor("L", m.a, m.ia, m.ja, partitions=256, reduced_solver="jacobi", precision=precision)
+    monkeypatch.setenv("SPIKE_PSTRSV_JACOBI_ITEMS", "1")
+    f1 = native.Factor("L", m.a, m.ia, m.ja, partitions=256, reduced_solver="jacobi", precision=precision)
+    assert f1.stats()["reduced_sweeps"] > 0
+    x0 = f0.solve(B, nrhs=nrhs)
+    x1 = f1.solve(B, nrhs=nrhs)
+    refs = [orc.ldsol(m.a, m.ia, m.ja, bj) for bj in B.reshape(nrhs, m.n)]
+    err = max(relerr(xj, r) for xj, r in zip(x1.reshape(nrhs, m.n), refs))
+    print(f"nrhs={nrhs} {precision} sweeps={f1.stats()['reduced_sweeps']} relerr {err:.3e}", flush=True)
+    assert err <= TOL
+    assert np.array_equal(x0, x1)
