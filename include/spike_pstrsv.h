@@ -290,6 +290,9 @@ typedef struct {
     double preproc_seconds;
     double autotune_seconds;
     int graph_nodes;          /* kernel launches per solve                     */
+    int lane_shift[4];        /* lanes per row at 1, 2, 4, 8 RHS slots: each   */
+                              /* matrix's row-length rule x 2^lane_shift       */
+                              /* (autotuned when lanes_per_row was 0)          */
 } spike_batch_stats_t;
 
 void spike_batch_options_default(spike_batch_options_t *o);

@@ -2048,6 +2048,9 @@ struct spike_batch_s {
     std::vector<spike_factor_s *> groups;  // block-diagonal factors
     int device = 0;
     int rpp[4] = {0, 0, 0, 0};             // rows per partition at 1, 2, 4, 8 RHS slots
+    // lanes per row at 1, 2, 4, 8 RHS slots, relative to each matrix's row
+    // length rule: G = rule x 2^lane_shift (within 1 .. 16); autotuned
+    int lane_shift[4] = {0, 0, 0, 0};
     double autotune_s = 0.0, preproc_s = 0.0;
     // captured solve graph for one (b, x) pair
     std::vector<hipStream_t> streams;
@@ -2183,7 +2186,8 @@ int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpe
 // Groups: right-hand-side slots (next power of two >= nrhs) x lanes per row
 // (from the p90 off-diagonal row length, or the option); a group is split
 // where it would pass 1.5e9 entries.
-std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vector<int> &idx, int lanes, bool fp32)
+std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vector<int> &idx, int lanes, bool fp32,
+                                   const int *lane_shift)
 {
     std::vector<GroupSpec> out;
     std::vector<std::pair<int, int>> key(idx.size());
@@ -2196,6 +2200,8 @@ std::vector<GroupSpec> plan_groups(const spike_batch_matrix_t *m, const std::vec
             std::vector<int> len(M.n);
             for (int r = 0; r < M.n; r++) len[r] = M.ia[r + 1] - M.ia[r] - 1;
             G = fp32 ? spike::choose_lanes(len) : spike::choose_block_lanes(len);
+            const int sh = lane_shift[__builtin_ctz((unsigned)slots)];
+            G = std::min(16, std::max(1, sh >= 0 ? G << sh : G >> -sh));
         }
         key[j] = {slots, G};
     }
@@ -2236,7 +2242,8 @@ int batch_run(spike_batch_s *bt, const double *b, double *x, hipStream_t s)
 int batch_build(spike_batch_s *bt, const spike_batch_matrix_t *m, const std::vector<int> &idx, const int *rpp,
                 int refine_steps)
 {
-    std::vector<GroupSpec> specs = plan_groups(m, idx, bt->opts.lanes_per_row, bt->opts.precision == SPIKE_PREC_FP32_REFINED);
+    std::vector<GroupSpec> specs =
+        plan_groups(m, idx, bt->opts.lanes_per_row, bt->opts.precision == SPIKE_PREC_FP32_REFINED, bt->lane_shift);
     for (const GroupSpec &g : specs) {
         spike_factor_s *f = nullptr;
         int rc = build_group(bt, m, g, rpp[__builtin_ctz((unsigned)g.slots)], bt->opts.precision, refine_steps,
@@ -2338,6 +2345,16 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
 {
     const int scale = bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 2 : 1;
     static const int cand[4][3] = {{512, 1024, 2048}, {256, 512, 1024}, {256, 512, 1024}, {128, 256, 512}};
+    // The sample holds about 4096 x the largest candidate's rows
+    // (SPIKE_PSTRSV_TUNE_SAMPLE: a multiple of 1024 x, default 4), so that it
+    // is bandwidth-bound like the full batch: with 1024 x the sample favoured
+    // small partitions, [1024, 512, 512, 256] rows at 1-8 slots, and the
+    // 4096-matrix batch took 29.3 ms per solve; with 4096 x it picks [2048,
+    // 1024, 512, 512] and half the lanes per row, 20.7 ms (FP64).
+    // SPIKE_PSTRSV_TUNE_LOG=1 prints the timings to stderr.
+    int sample_scale = 4;
+    if (const char *e = std::getenv("SPIKE_PSTRSV_TUNE_SAMPLE")) sample_scale = std::max(1, std::atoi(e));
+    const bool tlog = std::getenv("SPIKE_PSTRSV_TUNE_LOG") && std::atoi(std::getenv("SPIKE_PSTRSV_TUNE_LOG")) == 1;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -2355,7 +2372,7 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
             }
         }
         if (members.empty()) continue;
-        const int64_t target = 1024LL * cand[li][2] * scale;
+        const int64_t target = 1024LL * cand[li][2] * scale * sample_scale;
         const int64_t want = bt->opts.autotune_sample > 0 ? bt->opts.autotune_sample : INT64_MAX;
         const size_t step = rows > target ? (size_t)(rows / target) : 1;
         std::vector<spike_batch_matrix_t> sm;
@@ -2413,8 +2430,52 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
         // 38.5 ms per solve (34.8 with 256 for 8 RHS only) against 24.5 with
         // 512 for both.
         const double top = std::max(rate_of[0], std::max(rate_of[1], rate_of[2]));
+        if (tlog)
+            std::fprintf(stderr, "autotune: %d slots, %zu sample matrices, rows per partition %d / %d / %d: %.3e %.3e %.3e rows*RHS/s\n",
+                         1 << li, sm.size(), cand[li][0] * scale, cand[li][1] * scale, cand[li][2] * scale,
+                         rate_of[0], rate_of[1], rate_of[2]);
+        double base = -1.0;
         for (int ci = 0; ci < 3; ci++)
-            if (top > 0 && rate_of[ci] >= 0.90 * top) best[li] = cand[li][ci] * scale;
+            if (top > 0 && rate_of[ci] >= 0.90 * top) {
+                best[li] = cand[li][ci] * scale;
+                base = rate_of[ci];
+            }
+        // lanes per row (rows per wavefront step): half and twice each
+        // matrix's rule at the chosen partition size, timed the same way;
+        // a shift must beat the rule by 3%
+        bt->lane_shift[li] = 0;
+        if (bt->opts.lanes_per_row <= 0 && base > 0) {
+            double best_rate = base * 1.03;
+            for (int sh : {-1, 1}) {
+                tmp.release();
+                tmp.lane_shift[li] = sh;
+                int r4[4] = {best[li], best[li], best[li], best[li]};
+                if (batch_build(&tmp, sm.data(), sidx, r4, bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 1 : 0)) {
+                    t_err.clear();
+                    continue;
+                }
+                if ((rc = batch_run(&tmp, db, dx, nullptr))) break;
+                double rate = -1.0;
+                for (int round = 0; round < 3 && !rc; round++) {
+                    (void)hipEventRecord(e0, nullptr);
+                    for (int r = 0; r < 3 && !rc; r++) rc = batch_run(&tmp, db, dx, nullptr);
+                    (void)hipEventRecord(e1, nullptr);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (ms > 0) rate = std::max(rate, 3.0 * (double)len / (ms * 1e-3));
+                }
+                if (rc) break;
+                if (tlog)
+                    std::fprintf(stderr, "autotune: %d slots, lanes x 2^%d: %.3e rows*RHS/s (rule %.3e)\n", 1 << li, sh,
+                                 rate, base);
+                if (rate > best_rate) {
+                    best_rate = rate;
+                    bt->lane_shift[li] = sh;
+                }
+            }
+            tmp.lane_shift[li] = 0;
+        }
         tmp.release();
         (void)hipFree(db);
         (void)hipFree(dx);
@@ -2953,7 +3014,10 @@ int spike_batch_stats(spike_batch_t bt, spike_batch_stats_t *st)
         st->refine_steps = std::max(st->refine_steps, f->refine_steps);
         st->refine_est_err = std::max(st->refine_est_err, f->refine_est);
     }
-    for (int li = 0; li < 4; li++) st->rows_per_partition[li] = bt->rpp[li];
+    for (int li = 0; li < 4; li++) {
+        st->rows_per_partition[li] = bt->rpp[li];
+        st->lane_shift[li] = bt->lane_shift[li];
+    }
     st->preproc_seconds = bt->preproc_s;
     st->autotune_seconds = bt->autotune_s;
     st->graph_nodes = bt->graph_nodes;

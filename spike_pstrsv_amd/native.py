@@ -78,11 +78,12 @@ class BatchStats(C.Structure):
                 ("rows_rhs", C.c_int64), ("partitions", C.c_int64), ("n_reduced", C.c_int64),
                 ("rows_per_partition", C.c_int * 4), ("refine_steps", C.c_int), ("refine_est_err", C.c_double),
                 ("device_bytes", C.c_int64), ("preproc_seconds", C.c_double), ("autotune_seconds", C.c_double),
-                ("graph_nodes", C.c_int)]
+                ("graph_nodes", C.c_int), ("lane_shift", C.c_int * 4)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["rows_per_partition"] = list(self.rows_per_partition)
+        d["lane_shift"] = list(self.lane_shift)
         return d
 
 

@@ -107,7 +107,18 @@ def test_batch_autotuned(native):
     st = bt.stats()
     print("batch autotuned", st)
     assert all(r in (128, 256, 512, 1024, 2048) for r in st["rows_per_partition"])
+    assert all(sh in (-1, 0, 1) for sh in st["lane_shift"])
     b = packed_rhs(specs, bt.offsets, gen.SEED_BASE + 241)
+    check_batch(specs, mats, "L", bt.offsets, b, bt.solve(b))
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
+def test_batch_every_lane_count(native, lanes):
+    """Every lanes-per-row choice the autotuner may make solves to the bar."""
+    specs, mats = small_batch(24, seed=gen.SEED_BASE + 260)
+    bt = native.Batch("L", [(m.a, m.ia, m.ja, s["nrhs"]) for s, m in zip(specs, mats)], rows_per_partition=512,
+                      lanes_per_row=lanes)
+    b = packed_rhs(specs, bt.offsets, gen.SEED_BASE + 261)
     check_batch(specs, mats, "L", bt.offsets, b, bt.solve(b))
 
 
