@@ -59,14 +59,25 @@ def inblock_nnz(m, P: int) -> int:
     return int(np.sum(m.ja >= part * k))
 
 
+def _allowed_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+# Read before any OpenMP runtime starts: with OMP_PROC_BIND set, the runtime
+# binds the initial thread to one core, and the affinity mask read afterwards
+# holds that core only.
+ALLOWED_CPUS = _allowed_cpus()
+
+
 def host_cpu():
     """(threads to use, nproc, CPU model): min(nproc, 128) threads, capped by
-    the CPUs this process may run on and by OMP_NUM_THREADS when it is set."""
+    the CPUs this process may run on (read at start-up) and by
+    OMP_NUM_THREADS when it is set."""
     nproc = os.cpu_count() or 1
-    try:
-        allowed = len(os.sched_getaffinity(0))
-    except (AttributeError, OSError):
-        allowed = nproc
+    allowed = ALLOWED_CPUS
     threads = min(nproc, 128, allowed)
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
@@ -400,8 +411,18 @@ def main() -> int:
     m, P, desc = workload(args.workload)
     n = m.n
     b = gen.rhs(n, 1, seed=gen.SEED_BASE + 2)
+    t_before = native.timers()
+    t0 = time.perf_counter()
     fac = native.Factor("L", m.a, m.ia, m.ja, partitions=P)
+    create_s = time.perf_counter() - t0
+    t_after = native.timers()
     st = fac.stats()
+    # preprocessing phases (wall seconds of the library's phase timers):
+    # plan (partitions, dependents, step streams), spike build (device,
+    # synchronised), reduced-system assembly (device)
+    prep = {"factor_create_s": create_s, "preproc_s": st["preproc_seconds"],
+            "plan_s": t_after[1] - t_before[1], "spike_build_ms": 1e3 * (t_after[2] - t_before[2]),
+            "reduced_assembly_ms": 1e3 * (t_after[3] - t_before[3])}
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
     db = torch.from_numpy(b).cuda()
@@ -505,6 +526,7 @@ def main() -> int:
         "latency_ms": {"p50": statistics.median(lat), "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))],
                        "mall_flushed_p50": statistics.median(fl)},
         "wall_s_timed": wall,
+        "preprocess": prep,
     }
     # variants of the same matrix (not the metric line): 4 RHS, FP32 +
     # refinement; every column is checked against the oracle

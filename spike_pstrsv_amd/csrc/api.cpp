@@ -103,6 +103,8 @@ struct spike_factor_s {
     // sweep m launches the items ji_lists[ji_off[m-1] .. ji_off[m]).  Used at
     // RHS slot counts >= ji_min_slots (0: never).
     int ji_min_slots = 0, ji_kmax = 0;
+    int64_t ji_rnnz = 0;        // reduced-system entries (the item path's work gate)
+    bool ji_force = false;      // SPIKE_PSTRSV_JACOBI_ITEMS set: no work gate
     int *d_ji_items = nullptr, *d_ji_kcol = nullptr, *d_ji_need = nullptr, *d_ji_lists = nullptr;
     uint16_t *d_ji_lci = nullptr;
     std::vector<int> ji_off;
@@ -537,8 +539,10 @@ int setup_sync_free(spike_factor_s *f)
 // holds the distinct reduced columns its entries read.  Sweep m lists the items with a row that computes (need >=
 // m, flagged: the item gathers) or copies (need == m-1, or need == 0 at
 // m == 2).  The item path serves solves of at least SPIKE_PSTRSV_JACOBI_ITEMS
-// RHS slots (default 4; 0 turns it off): with several right-hand sides one
-// gather per column replaces one per entry.  Measured on the 4096-matrix
+// RHS slots (default 4; 0 turns it off; a value set there also drops the
+// size gate) whose reduced system times the slot count reaches 2^25
+// entries: with several right-hand sides one gather per column replaces one
+// per entry, and small reduced systems keep the row sweeps.  Measured on the 4096-matrix
 // batch (512-row partitions, kernel time per solve): 8 slots 14.8 -> 10.9
 // ms, 4 slots 5.7 -> 4.4 ms, 2 slots 2.7 -> 3.4 ms; one RHS on the 1M-row
 // banded matrix 47 -> 60 us.
@@ -548,7 +552,11 @@ int build_jacobi_items(spike_factor_s *f, const std::vector<int> &hrp, const std
     const Plan &pl = f->plan;
     f->ji_min_slots = 0;
     int min_slots = 4;
-    if (const char *e = std::getenv("SPIKE_PSTRSV_JACOBI_ITEMS")) min_slots = std::max(0, std::atoi(e));
+    f->ji_force = false;
+    if (const char *e = std::getenv("SPIKE_PSTRSV_JACOBI_ITEMS")) {
+        min_slots = std::max(0, std::atoi(e));
+        f->ji_force = true;
+    }
     if (min_slots == 0 || f->sweeps <= 0) return SPIKE_OK;
     constexpr int kTarget = 2048, kColCap = 256;
     const int nred = (int)pl.dep_rows.size();
@@ -610,6 +618,7 @@ int build_jacobi_items(spike_factor_s *f, const std::vector<int> &hrp, const std
         (rc = f->upload(&f->d_ji_lists, lists)))
         return rc;
     f->ji_kmax = kmax;
+    f->ji_rnnz = (int64_t)rnnz;
     f->ji_min_slots = min_slots;
     return SPIKE_OK;
 }
@@ -1136,6 +1145,8 @@ struct FactorScalars {
     int flip, n, P, kmax, nlev, red_in_lds, sweeps, ring, cpl_fused, fuse_at[4], tile_at[4], max_nr, cmax;
     int fp32, refine_steps, method, caller_parts, nlev_full, lev_G, nres, G, M, R, fwd, fused, nchunks;
     int ji_min_slots, ji_kmax;
+    int64_t ji_rnnz;
+    int ji_force;
     int64_t nnz, spike_area;
     double refine_rate, refine_est;
     uint64_t lds_needed;
@@ -1229,6 +1240,8 @@ int factor_save(spike_factor_s *f, const char *path)
     sc.nchunks = f->nchunks;
     sc.ji_min_slots = f->ji_min_slots;
     sc.ji_kmax = f->ji_kmax;
+    sc.ji_rnnz = f->ji_rnnz;
+    sc.ji_force = f->ji_force;
     sc.nnz = pl.nnz;
     sc.spike_area = pl.spike_area;
     sc.refine_rate = f->refine_rate;
@@ -1331,6 +1344,8 @@ int factor_load(spike_factor_s *f, const char *path)
     f->nchunks = sc.nchunks;
     f->ji_min_slots = sc.ji_min_slots;
     f->ji_kmax = sc.ji_kmax;
+    f->ji_rnnz = sc.ji_rnnz;
+    f->ji_force = sc.ji_force != 0;
     (void)hipDeviceGetAttribute(&f->ncu, hipDeviceAttributeMultiprocessorCount, f->device);
     (void)hipDeviceGetAttribute(&f->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, f->device);
     (void)hipGetLastError();
@@ -1618,7 +1633,11 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
         const double *xr = f->d_y;
         if (f->sweeps > 0) {
             const double *xin = f->d_y;
+            // items where the sweeps are bandwidth-bound (at least 2^25
+            // entry x RHS-slot products per sweep); small reduced systems
+            // keep the row sweeps, whose launches have one round trip less
             const bool items = f->ji_min_slots > 0 && slots >= f->ji_min_slots &&
+                               (f->ji_force || f->ji_rnnz * slots >= ((int64_t)1 << 25)) &&
                                spike::jacobi_items_lds(f->ji_kmax, slots) <= spike::block_solve_lds_limit();
             spike::JacobiItems ji{f->d_ji_items, f->d_ji_kcol, f->d_rrp, f->d_ji_lci, f->d_rv, f->d_ji_need, f->d_y,
                                   f->ji_kmax};
