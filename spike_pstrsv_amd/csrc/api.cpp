@@ -96,6 +96,9 @@ struct spike_factor_s {
     int nlev = 0;
     bool red_in_lds = false;
     int sweeps = 0;             // > 0: Jacobi sweeps, 0: exact level solve
+    // sweeps an FP32 factor's solves run: the inner solves of refinement
+    // need the reduced solve only to about FP32 accuracy (tail bound 1e-8)
+    int sweeps_fp32 = 0;
     std::vector<int> sweep_off; // sweep m: int4 row records off[m-1] .. off[m]
     int *d_sweep_rows = nullptr;
     // Jacobi sweeps over items (spike::launch_jacobi_items): the items, their
@@ -955,7 +958,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     // mx_J < 1 the terms beyond J sum to at most
     // mx_J / (1 - mx_J) * (1 + mx_1 + ... + mx_{J-1}).  W is nilpotent, so
     // terms past its level count vanish exactly.
-    int sweeps = -1;
+    int sweeps = -1, sweeps32 = -1;
     double jbound = 0.0;
     std::vector<int> row_sweeps(nred, 0);    // sweeps each row needs
     if (rnnz > 0) {
@@ -994,6 +997,20 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         // banded matrix it drops a seventh sweep of 320 rows, 4.5 us, and
         // leaves the solution's error at 9.3e-16 unchanged)
         constexpr double kTol = 2e-15;
+        {
+            // FP32 factors: the smallest S whose tail bound is <= 1e-8
+            double t = beyond;
+            for (int j = 0; j < J; j++) t += mx[j];
+            sweeps32 = J;
+            for (int S = 0; S < J; S++) {
+                if (t <= 1e-8) {
+                    sweeps32 = S;
+                    break;
+                }
+                t -= mx[S];
+            }
+            if (beyond > 1e-8) sweeps32 = -1;
+        }
         double tail = beyond;
         if (tail <= kTol) {
             sweeps = J;
@@ -1029,6 +1046,8 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         return fail(SPIKE_ERR_UNSUPPORTED, "Jacobi sweeps do not reach the reduced-system tolerance in 64 terms");
     if (mode == SPIKE_REDUCED_AUTO) mode = (rnnz > 0 && t_jacobi < t_exact) ? SPIKE_REDUCED_JACOBI : SPIKE_REDUCED_EXACT;
     f->sweeps = (mode == SPIKE_REDUCED_JACOBI && rnnz > 0) ? std::max(sweeps, 1) : 0;
+    f->sweeps_fp32 = f->sweeps;
+    if (f->fp32 && f->sweeps > 0 && sweeps32 >= 0) f->sweeps_fp32 = std::max(1, std::min(sweeps32, f->sweeps));
     if (f->sweeps > 0) {
         // sweep m computes rows that need >= m sweeps and copies rows that
         // finished at m-1 (rows finished at 0 are copied in sweeps 1 and 2),
@@ -1136,7 +1155,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 // File layout: magic, a block of scalars, then tagged records
 // {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
 // stored by content; workspaces only by size (re-allocated on load).
-constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '6'};
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '7'};
 
 struct FactorScalars {
     spike_options_t opts;
@@ -1144,7 +1163,7 @@ struct FactorScalars {
     char uplo;
     int flip, n, P, kmax, nlev, red_in_lds, sweeps, ring, cpl_fused, fuse_at[4], tile_at[4], max_nr, cmax;
     int fp32, refine_steps, method, caller_parts, nlev_full, lev_G, nres, G, M, R, fwd, fused, nchunks;
-    int ji_min_slots, ji_kmax;
+    int ji_min_slots, ji_kmax, sweeps_fp32;
     int64_t ji_rnnz;
     int ji_force;
     int64_t nnz, spike_area;
@@ -1240,6 +1259,7 @@ int factor_save(spike_factor_s *f, const char *path)
     sc.nchunks = f->nchunks;
     sc.ji_min_slots = f->ji_min_slots;
     sc.ji_kmax = f->ji_kmax;
+    sc.sweeps_fp32 = f->sweeps_fp32;
     sc.ji_rnnz = f->ji_rnnz;
     sc.ji_force = f->ji_force;
     sc.nnz = pl.nnz;
@@ -1344,6 +1364,7 @@ int factor_load(spike_factor_s *f, const char *path)
     f->nchunks = sc.nchunks;
     f->ji_min_slots = sc.ji_min_slots;
     f->ji_kmax = sc.ji_kmax;
+    f->sweeps_fp32 = sc.sweeps_fp32;
     f->ji_rnnz = sc.ji_rnnz;
     f->ji_force = sc.ji_force != 0;
     (void)hipDeviceGetAttribute(&f->ncu, hipDeviceAttributeMultiprocessorCount, f->device);
@@ -1641,7 +1662,8 @@ int solve_pass(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, 
                                spike::jacobi_items_lds(f->ji_kmax, slots) <= spike::block_solve_lds_limit();
             spike::JacobiItems ji{f->d_ji_items, f->d_ji_kcol, f->d_rrp, f->d_ji_lci, f->d_rv, f->d_ji_need, f->d_y,
                                   f->ji_kmax};
-            for (int m = 1; m <= f->sweeps; m++) {
+            const int S = f->fp32 ? f->sweeps_fp32 : f->sweeps;
+            for (int m = 1; m <= S; m++) {
                 double *xout = (m & 1) ? f->d_xa : f->d_xb;
                 if (items) {
                     const int o0 = f->ji_off[m - 1], o1 = f->ji_off[m];
