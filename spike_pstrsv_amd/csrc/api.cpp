@@ -539,16 +539,18 @@ int setup_sync_free(spike_factor_s *f)
 // each partition cut into runs of consecutive rows of at most about 2048
 // entries and 256 distinct columns (the column values in LDS stay small
 // enough for eight workgroups per CU at 8 RHS slots); an item's column list
-// holds the distinct reduced columns its entries read.  Sweep m lists the items with a row that computes (need >=
-// m, flagged: the item gathers) or copies (need == m-1, or need == 0 at
-// m == 2).  The item path serves solves of at least SPIKE_PSTRSV_JACOBI_ITEMS
-// RHS slots (default 4; 0 turns it off; a value set there also drops the
-// size gate) whose reduced system times the slot count reaches 2^25
-// entries: with several right-hand sides one gather per column replaces one
-// per entry, and small reduced systems keep the row sweeps.  Measured on the 4096-matrix
-// batch (512-row partitions, kernel time per solve): 8 slots 14.8 -> 10.9
-// ms, 4 slots 5.7 -> 4.4 ms, 2 slots 2.7 -> 3.4 ms; one RHS on the 1M-row
-// banded matrix 47 -> 60 us.
+// holds the distinct reduced columns its entries read.  Sweep m lists the
+// items with a row that computes (need >= m; flagged: the item gathers) or
+// copies (need == m-1, or need == 0 at m == 2).
+//
+// The item path serves solves of at least SPIKE_PSTRSV_JACOBI_ITEMS RHS
+// slots (default 4; 0 turns it off; a value set there also drops the size
+// gate) whose reduced system times the slot count reaches 2^25 entries: with
+// several right-hand sides one gather per column replaces one per entry, and
+// small reduced systems keep the row sweeps.  Measured on the 4096-matrix
+// batch (512-row partitions, kernel time per solve): 8 slots 14.8 -> 10.9 ms,
+// 4 slots 5.7 -> 4.4 ms, 2 slots 2.7 -> 3.4 ms; one RHS on the 1M-row banded
+// matrix 47 -> 60 us.
 int build_jacobi_items(spike_factor_s *f, const std::vector<int> &hrp, const std::vector<int> &hci,
                        const std::vector<int> &row_sweeps)
 {
