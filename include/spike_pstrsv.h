@@ -115,7 +115,11 @@ typedef struct {
                               /* 2: every such repeated solve is captured      */
                               /* (least host time per solve; the serving loop  */
                               /* uses it); 0: never                            */
-    int autotune;             /* 1: time the RHS tile candidates at creation   */
+    int autotune;             /* 1: time the RHS tile candidates at creation;  */
+                              /* 2: also build and time half and twice the     */
+                              /* automatic partition count (partitions == 0)   */
+                              /* and lanes per row (lanes_per_row == 0) over   */
+                              /* one-RHS solves, keeping the fastest           */
 } spike_options_t;
 
 /* Solve methods. */

@@ -2576,6 +2576,119 @@ int batch_capture(spike_batch_s *bt, const double *b, double *x)
     return SPIKE_OK;
 }
 
+// Wall-clock milliseconds of one 1-RHS device solve of f: two warm-up
+// solves, then the fastest of three rounds of three.
+int time_factor(spike_factor_s *f, const double *db, double *dx, double *ms_out)
+{
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int n = f->plan.n;
+    int rc = SPIKE_OK;
+    for (int r = 0; r < 2 && !rc; r++) rc = factor_solve(f, 1, db, n, dx, n, nullptr, false);
+    float best = 1e30f;
+    for (int round = 0; round < 3 && !rc; round++) {
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < 3 && !rc; r++) rc = factor_solve(f, 1, db, n, dx, n, nullptr, false);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        best = std::min(best, ms / 3.0f);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = best;
+    return rc;
+}
+
+// spike_factor_create with autotune >= 2: the rule's factor, then half and
+// twice its partition count (when partitions is 0) and, at the faster count,
+// half and twice its lanes per row (when lanes_per_row is 0), each built and
+// timed over one-RHS solves; a candidate replaces the current choice when
+// it is 3% faster.  Level-method factors are kept as built.
+int factor_create_tuned(spike_factor_s **out, char uplo, int n, const double *a, const int *ia, const int *ja,
+                        const spike_options_t &o)
+{
+    auto build = [&](int P, int G, spike_factor_s **res) -> int {
+        spike_factor_s *f = new spike_factor_s();
+        f->opts = o;
+        f->opts.autotune = 1;
+        f->opts.partitions = P;
+        f->opts.lanes_per_row = G;
+        int rc = factor_build(f, uplo, n, a, ia, ja);
+        if (rc != SPIKE_OK) {
+            f->release();
+            delete f;
+            *res = nullptr;
+            return rc;
+        }
+        *res = f;
+        return SPIKE_OK;
+    };
+    spike_factor_s *best = nullptr;
+    int rc = build(o.partitions, o.lanes_per_row, &best);
+    if (rc != SPIKE_OK) return rc;
+    if (best->method == SPIKE_METHOD_LEVELS) {
+        *out = best;
+        return SPIKE_OK;
+    }
+    double *db = nullptr, *dx = nullptr;
+    if (hipMalloc((void **)&db, sizeof(double) * n) != hipSuccess ||
+        hipMalloc((void **)&dx, sizeof(double) * n) != hipSuccess) {
+        (void)hipGetLastError();
+        if (db) (void)hipFree(db);
+        *out = best;   // untuned, but valid
+        return SPIKE_OK;
+    }
+    {
+        std::vector<double> hb(n);
+        for (int i = 0; i < n; i++) hb[i] = std::sin(0.37 * (double)i);
+        (void)hipMemcpy(db, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    }
+    double best_ms = 0.0;
+    rc = time_factor(best, db, dx, &best_ms);
+    auto consider = [&](int P, int G) {
+        spike_factor_s *c = nullptr;
+        if (rc != SPIKE_OK || build(P, G, &c) != SPIKE_OK) {
+            t_err.clear();   // a candidate that does not build is skipped
+            return;
+        }
+        double ms = 0.0;
+        if (c->method == SPIKE_METHOD_SPIKE && time_factor(c, db, dx, &ms) == SPIKE_OK && ms < best_ms * 0.97) {
+            best->release();
+            delete best;
+            best = c;
+            best_ms = ms;
+            return;
+        }
+        t_err.clear();
+        c->release();
+        delete c;
+    };
+    if (rc == SPIKE_OK && o.partitions == 0) {
+        const int P0 = best->plan.P, G0 = best->steps.G;
+        if (P0 / 2 >= 1) consider(P0 / 2, o.lanes_per_row > 0 ? o.lanes_per_row : G0);
+        if (P0 * 2 <= n) consider(P0 * 2, o.lanes_per_row > 0 ? o.lanes_per_row : G0);
+    }
+    if (rc == SPIKE_OK && o.lanes_per_row == 0 && !best->steps.fwd) {
+        const int P1 = best->plan.P, G1 = best->steps.G;
+        if (G1 / 2 >= 1) consider(P1, G1 / 2);
+        if (G1 * 2 <= 16) consider(P1, G1 * 2);
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    if (rc != SPIKE_OK) {
+        std::string keep = t_err;
+        best->release();
+        delete best;
+        t_err = keep;
+        return rc;
+    }
+    *out = best;
+    return SPIKE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2610,6 +2723,14 @@ int spike_factor_create(spike_factor_t *out, char uplo, int n, const double *a, 
     if (opts) o = *opts;
     if (o.precision != SPIKE_PREC_FP64 && o.precision != SPIKE_PREC_FP32_REFINED)
         return fail(SPIKE_ERR_ARG, "unknown precision mode");
+    if (o.autotune >= 2 && (o.partitions == 0 || o.lanes_per_row == 0)) {
+        spike_factor_s *t = nullptr;
+        const int rc = factor_create_tuned(&t, uplo, n, a, ia, ja, o);
+        if (rc != SPIKE_OK) return rc;
+        *out = t;
+        t_err.clear();
+        return SPIKE_OK;
+    }
     spike_factor_s *f = new spike_factor_s();
     f->opts = o;
     int rc = factor_build(f, uplo, n, a, ia, ja);

@@ -234,7 +234,7 @@ METHODS = {"auto": 0, "spike": 1, "levels": 2}
 
 def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0, verbose: bool = False,
             reduced_solver: str = "auto", precision: str = "fp64", refine_steps: int = -1,
-            refine_tol: float = 0.0, method: str = "auto", use_graph: int = 1) -> Options:
+            refine_tol: float = 0.0, method: str = "auto", use_graph: int = 1, autotune: int = 1) -> Options:
     o = Options()
     lib.spike_options_default(C.byref(o))
     o.partitions = partitions
@@ -247,6 +247,7 @@ def options(partitions: int = 0, drop_tol: float = -1.0, lanes_per_row: int = 0,
     o.refine_tol = refine_tol
     o.method = METHODS[method]
     o.use_graph = int(use_graph)
+    o.autotune = int(autotune)
     return o
 
 
@@ -256,12 +257,12 @@ class Factor:
     def __init__(self, uplo: str, a, ia, ja, partitions: int = 0, drop_tol: float = -1.0,
                  lanes_per_row: int = 0, verbose: bool = False, reduced_solver: str = "auto",
                  precision: str = "fp64", refine_steps: int = -1, refine_tol: float = 0.0,
-                 method: str = "auto", starts=None, use_graph: int = 1):
+                 method: str = "auto", starts=None, use_graph: int = 1, autotune: int = 1):
         a, ia, ja = _csr(a, ia, ja)
         self.n = len(ia) - 1
         h = _vp()
         o = options(partitions, drop_tol, lanes_per_row, verbose, reduced_solver, precision, refine_steps,
-                    refine_tol, method, use_graph)
+                    refine_tol, method, use_graph, autotune)
         if starts is None:
             _check(lib.spike_factor_create(C.byref(h), uplo.encode()[:1], self.n, a.ctypes.data, ia.ctypes.data,
                                            ja.ctypes.data, C.byref(o)))

@@ -357,3 +357,20 @@ def test_jacobi_items_match_row_sweeps(native, monkeypatch, nrhs, precision):
     print(f"nrhs={nrhs} {precision} sweeps={f1.stats()['reduced_sweeps']} relerr {err:.3e}", flush=True)
     assert err <= TOL
     assert np.array_equal(x0, x1)
+
+
+def test_autotuned_partitions_and_lanes(native):
+    """autotune = 2 builds half and twice the automatic partition count and
+    lanes per row, times them and keeps the fastest; the result solves to
+    the bar."""
+    m = gen.banded(1 << 18, seed=gen.SEED_BASE + 45)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 46)
+    s0 = native.Factor("L", m.a, m.ia, m.ja).stats()
+    f2 = native.Factor("L", m.a, m.ia, m.ja, autotune=2)
+    s2 = f2.stats()
+    print("rule", s0["partitions"], s0["lanes_per_row"], "tuned", s2["partitions"], s2["lanes_per_row"], flush=True)
+    assert s2["partitions"] in (s0["partitions"] // 2, s0["partitions"], 2 * s0["partitions"])
+    assert s2["lanes_per_row"] in (max(1, s0["lanes_per_row"] // 2), s0["lanes_per_row"],
+                                   min(16, 2 * s0["lanes_per_row"]))
+    x = f2.solve(b)
+    assert relerr(x, orc.ldsol(m.a, m.ia, m.ja, b)) <= TOL
