@@ -17,7 +17,9 @@ per step): algorithmic bytes per launch = 12 B per in-block nonzero (FP64
 value + int32 column) + 4 B per row pointer + 8 B read of b + 8 B write per
 row, divided by the kernel's average device time measured with HIP events on
 the launch stream.  `cpu_baseline` times the CPU restatement of the
-reference's Spike algorithm (OpenMP, same partitions) on a bounded sample.
+reference's Spike algorithm (OpenMP, same partitions) on a bounded sample,
+and beside it (`competitors`) the reference harness's MKL triangular solves
+on the same system when MKL's runtime library loads.
 """
 from __future__ import annotations
 
@@ -126,8 +128,88 @@ def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
         "nproc": nproc,
         "cpu_model": model,
         "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")},
+        "competitors": mkl_competitors(m, b, ref, threads, min(seconds / 2.0, 5.0)),
     }
     return cpu, rel
+
+
+MKL_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libmkl_rt.so.1")
+
+
+def mkl_competitors(m, b, ref, threads: int, seconds: float):
+    """Times the reference harness's CPU competitors (test/runtime_analysis.c:
+    67-87): MKL's inspector-executor `mkl_sparse_d_trsv` (analysis with
+    `mkl_sparse_set_sv_hint` + `mkl_sparse_optimize` outside the timed region,
+    as test/runtime_analysis.c:358-362) and the zero-based
+    `mkl_cspblas_dcsrtrsv`, on the same matrix and right-hand side, when
+    MKL's runtime library loads.  Each gets a bounded sample of `seconds`."""
+    import ctypes
+
+    import numpy as np
+
+    os.environ.setdefault("MKL_THREADING_LAYER", "GNU")   # share the restatement's libgomp
+    lib, path = None, None
+    for cand in MKL_CANDIDATES:
+        try:
+            lib, path = ctypes.CDLL(cand), cand
+            break
+        except OSError:
+            continue
+    if lib is None:
+        return {"available": False, "reason": "libmkl_rt not found"}
+    try:
+        lib.MKL_Set_Num_Threads(ctypes.c_int(threads))
+    except AttributeError:
+        pass
+    n = m.n
+    a = np.ascontiguousarray(m.a, dtype=np.float64)
+    ia = np.ascontiguousarray(m.ia, dtype=np.int32)
+    ja = np.ascontiguousarray(m.ja, dtype=np.int32)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    ptr = lambda v: v.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+
+    def sample(call):
+        x = np.zeros(n)
+        call(x)                                           # warm-up
+        count, t0 = 0, time.perf_counter()
+        while True:
+            call(x)
+            count += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or count >= 5000:
+                break
+        return {"value": n * count / el, "unit": UNIT, "solves": count, "seconds": round(el, 3),
+                "rel_err_vs_oracle": reldiff(x, ref)}
+
+    out = {"available": True, "library": path, "threads": int(lib.MKL_Get_Max_Threads()),
+           "threading_layer": os.environ.get("MKL_THREADING_LAYER")}
+
+    class Descr(ctypes.Structure):
+        _fields_ = [("type", ctypes.c_int), ("mode", ctypes.c_int), ("diag", ctypes.c_int)]
+
+    # sparse_matrix_type TRIANGULAR = 23, fill LOWER = 40, diag NON_UNIT = 50,
+    # operation NON_TRANSPOSE = 10, index base ZERO = 0 (mkl_spblas.h)
+    descr = Descr(23, 40, 50)
+    A = ctypes.c_void_p()
+    rs, re_ = ia[:-1].copy(), ia[1:].copy()
+    st = lib.mkl_sparse_d_create_csr(ctypes.byref(A), 0, n, n, ptr(rs), ptr(re_), ptr(ja), ptr(a))
+    if st == 0:
+        lib.mkl_sparse_set_sv_hint.argtypes = [ctypes.c_void_p, ctypes.c_int, Descr, ctypes.c_int]
+        lib.mkl_sparse_d_trsv.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, Descr, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+        st = lib.mkl_sparse_set_sv_hint(A, 10, descr, 1000) or lib.mkl_sparse_optimize(A)
+    if st == 0:
+        out["mkl_sparse_d_trsv"] = sample(lambda x: lib.mkl_sparse_d_trsv(10, 1.0, A, descr, ptr(bb), ptr(x)))
+    else:
+        out["mkl_sparse_d_trsv"] = {"error": f"sparse_status_t {st}"}
+    if A:
+        lib.mkl_sparse_destroy(A)
+    nn = ctypes.c_int(n)
+    lib.mkl_cspblas_dcsrtrsv.restype = None
+    out["mkl_cspblas_dcsrtrsv"] = sample(
+        lambda x: lib.mkl_cspblas_dcsrtrsv(b"L", b"N", b"N", ctypes.byref(nn), ptr(a), ptr(ia), ptr(ja), ptr(bb),
+                                           ptr(x)))
+    return out
 
 
 def oracle_err(m, B, X, uplo="L"):

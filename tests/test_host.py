@@ -272,3 +272,23 @@ def test_factor_load_rejects_foreign_file(tmp_path):
         native.Factor.load(p)
     with pytest.raises(native.SpikeError):
         native.Factor.load(tmp_path / "missing.spk")
+
+
+def test_bench_mkl_competitors_match_oracle():
+    """The CPU competitors bench.py reports (MKL's inspector-executor and
+    cspblas triangular solves) solve the same system as the oracle's
+    sequential substitution; skipped where MKL's runtime does not load."""
+    import sys
+
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import bench
+
+    m = gen.banded(20000, 64, 16, 0.02, gen.SEED_BASE + 2)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 3).ravel().copy()
+    ref = orc.ldsol(m.a, m.ia, m.ja, b)
+    out = bench.mkl_competitors(m, b, ref, 2, 0.05)
+    if not out["available"]:
+        pytest.skip(out["reason"])
+    for name in ("mkl_sparse_d_trsv", "mkl_cspblas_dcsrtrsv"):
+        assert out[name]["rel_err_vs_oracle"] <= 1e-12, (name, out[name])
+        assert out[name]["solves"] >= 1
