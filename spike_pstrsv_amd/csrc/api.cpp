@@ -968,15 +968,25 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         std::vector<double> r(nred, 1.0);
         std::vector<double> mx;              // mx[j-1] = max_q T[j-1][q]
         bool exact_end = false;
+        // row chunks of about 2^16 entries on the host threads
+        const int64_t grain = std::max<int64_t>(256, (int64_t)nred * 65536 / std::max<int64_t>(rnnz, 1));
+        std::vector<double> cmax((size_t)std::min<int64_t>((nred + grain - 1) / grain, 1 << 20));
         for (int j = 1; j <= 64; j++) {
-            std::vector<double> r2(nred);
+            std::vector<double> r2;
+            spike::resize_large(r2, nred);
+            std::fill(cmax.begin(), cmax.end(), 0.0);
+            spike::parallel_chunks(nred, grain, [&](int c, int64_t lo, int64_t hi) {
+                double cm = 0.0;
+                for (int64_t q = lo; q < hi; q++) {
+                    double acc = 0.0;
+                    for (int e = hrp[q]; e < hrp[q + 1]; e++) acc += std::fabs(hv[e]) * r[hci[e]];
+                    r2[q] = acc;
+                    cm = std::max(cm, acc);
+                }
+                cmax[c] = cm;
+            });
             double m = 0.0;
-            for (int q = 0; q < nred; q++) {
-                double acc = 0.0;
-                for (int e = hrp[q]; e < hrp[q + 1]; e++) acc += std::fabs(hv[e]) * r[hci[e]];
-                r2[q] = acc;
-                m = std::max(m, acc);
-            }
+            for (double v : cmax) m = std::max(m, v);
             mx.push_back(m);
             T.push_back(r2);
             r.swap(r2);
@@ -1024,16 +1034,18 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
                 jbound = tail;
             }
             // per row: terms past J are bounded by T_J[q] * mass
-            for (int q = 0; q < nred; q++) {
-                double t = exact_end ? 0.0 : T[J - 1][q] * mass;
-                int m = J;
-                for (int S = J - 1; S >= 0; S--) {
-                    t += T[S][q];
-                    if (t > kTol) break;
-                    m = S;
+            spike::parallel_chunks(nred, 1 << 16, [&](int, int64_t lo, int64_t hi) {
+                for (int64_t q = lo; q < hi; q++) {
+                    double t = exact_end ? 0.0 : T[J - 1][q] * mass;
+                    int m = J;
+                    for (int S = J - 1; S >= 0; S--) {
+                        t += T[S][q];
+                        if (t > kTol) break;
+                        m = S;
+                    }
+                    row_sweeps[q] = std::min(m, sweeps);
                 }
-                row_sweeps[q] = std::min(m, sweeps);
-            }
+            });
         }
     }
     // cost model (microseconds): the level solve runs in one workgroup,
@@ -2170,8 +2182,11 @@ int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpe
         nnz += m[i].ia[m[i].n];
     }
     if (rows >= INT32_MAX || nnz > INT32_MAX) return fail(SPIKE_ERR_UNSUPPORTED, "batch group exceeds 2^31-1 entries");
-    std::vector<int> ia((size_t)rows + 1), ja((size_t)nnz), starts{0};
-    std::vector<double> a((size_t)nnz);
+    std::vector<int> ia, ja, starts{0};
+    std::vector<double> a;
+    spike::resize_large(ia, (size_t)rows + 1);
+    spike::resize_large(ja, (size_t)nnz);
+    spike::resize_large(a, (size_t)nnz);
     spike_factor_s *f = new spike_factor_s();
     spike_options_default(&f->opts);
     f->opts.precision = precision;
@@ -2184,15 +2199,22 @@ int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpe
     f->vec_len = bt->off[bt->count];
     const size_t vb = precision == SPIKE_PREC_FP32_REFINED ? sizeof(float) : sizeof(double);
     const int kcap = std::min(kmax_limit_for_device(), (int)(spike::block_solve_lds_limit() / (vb * g.slots)) - spike::XS_PAD);
-    int64_t r0 = 0;
+    // each matrix's rows and entries start at the sums over the ones before
+    // it, so the matrices are copied in parallel
+    std::vector<int64_t> row0(g.mats.size() + 1, 0), ent0(g.mats.size() + 1, 0);
+    for (size_t k = 0; k < g.mats.size(); k++) {
+        row0[k + 1] = row0[k] + m[g.mats[k]].n;
+        ent0[k + 1] = ent0[k] + m[g.mats[k]].ia[m[g.mats[k]].n];
+    }
     ia[0] = 0;
-    for (int i : g.mats) {
-        const spike_batch_matrix_t &M = m[i];
+    spike::parallel_for((int)g.mats.size(), [&](int k) {
+        const spike_batch_matrix_t &M = m[g.mats[k]];
         const int n = M.n;
+        const int64_t r0 = row0[k];
+        int64_t p = ent0[k];
         for (int r = 0; r < n; r++) {
             const int src = flip ? n - 1 - r : r;
             const int b0 = M.ia[src], b1 = M.ia[src + 1];
-            int64_t p = ia[r0 + r];
             if (!flip)
                 for (int e = b0; e < b1; e++, p++) {
                     ja[p] = (int)r0 + M.ja[e];
@@ -2205,6 +2227,11 @@ int build_group(spike_batch_s *bt, const spike_batch_matrix_t *m, const GroupSpe
                 }
             ia[r0 + r + 1] = (int)p;
         }
+    });
+    int64_t r0 = 0;
+    for (int i : g.mats) {
+        const spike_batch_matrix_t &M = m[i];
+        const int n = M.n;
         f->mats.push_back(spike_factor_s::Mat{(int)r0, n, M.nrhs, bt->off[i]});
         f->batch_max_nrhs = std::max(f->batch_max_nrhs, M.nrhs);
         int P = std::max(1, (n + rpp / 2) / std::max(rpp, 1));

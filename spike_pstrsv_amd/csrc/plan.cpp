@@ -10,34 +10,10 @@ namespace spike {
 
 namespace {
 
-template <class F>
-void parallel_for(int count, F &&fn)
+// First defect of rows [lo, hi), or "".
+std::string validate_rows(char uplo, int n, const int *ia, const int *ja, int lo, int hi)
 {
-    int nt = (int)std::thread::hardware_concurrency();
-    if (nt < 1) nt = 1;
-    if (nt > 32) nt = 32;
-    if (count < 2 * nt) nt = std::max(1, std::min(nt, count));
-    if (nt == 1) {
-        for (int i = 0; i < count; i++) fn(i);
-        return;
-    }
-    std::atomic<int> next{0};
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++)
-        th.emplace_back([&]() {
-            for (;;) {
-                int i = next.fetch_add(1);
-                if (i >= count) break;
-                fn(i);
-            }
-        });
-    for (auto &x : th) x.join();
-}
-
-std::string validate(char uplo, int n, const int *ia, const int *ja)
-{
-    if (ia[0] != 0) return "ia[0] must be 0";
-    for (int i = 0; i < n; i++) {
+    for (int i = lo; i < hi; i++) {
         if (ia[i + 1] <= ia[i]) return "row " + std::to_string(i) + " is empty (diagonal missing)";
         for (int j = ia[i]; j < ia[i + 1]; j++) {
             if (ja[j] < 0 || ja[j] >= n) return "column index out of range in row " + std::to_string(i);
@@ -50,6 +26,19 @@ std::string validate(char uplo, int n, const int *ia, const int *ja)
             if (ja[ia[i]] != i) return "row " + std::to_string(i) + ": diagonal must be the first entry ('U')";
         }
     }
+    return "";
+}
+
+// The first defect in row order (the message a row-by-row scan gives).
+std::string validate(char uplo, int n, const int *ia, const int *ja)
+{
+    if (ia[0] != 0) return "ia[0] must be 0";
+    std::vector<std::string> err((size_t)std::min<int64_t>((n + 65535) / 65536, 1 << 20));
+    const int nc = parallel_chunks(n, 65536, [&](int c, int64_t lo, int64_t hi) {
+        err[c] = validate_rows(uplo, n, ia, ja, (int)lo, (int)hi);
+    });
+    for (int c = 0; c < nc; c++)
+        if (!err[c].empty()) return err[c];
     return "";
 }
 
@@ -245,13 +234,15 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     pl.n = n;
     pl.P = P;
     pl.nnz = nnz;
-    pl.ia.resize(n + 1);
-    pl.ja.resize(nnz);
-    pl.a.resize(nnz);
+    resize_large(pl.ia, n + 1);
+    resize_large(pl.ja, nnz);
+    resize_large(pl.a, nnz);
     if (!pl.flip) {
         std::copy(ia, ia + n + 1, pl.ia.begin());
-        std::copy(ja, ja + nnz, pl.ja.begin());
-        std::copy(a, a + nnz, pl.a.begin());
+        parallel_chunks(nnz, 1 << 20, [&](int, int64_t lo, int64_t hi) {
+            std::copy(ja + lo, ja + hi, pl.ja.begin() + lo);
+            std::copy(a + lo, a + hi, pl.a.begin() + lo);
+        });
     } else {
         // U = J L J: row i of L is row n-1-i of U reversed, columns mirrored
         pl.ia[0] = 0;
@@ -259,12 +250,14 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
             int src = n - 1 - i;
             pl.ia[i + 1] = pl.ia[i] + (ia[src + 1] - ia[src]);
         }
-        parallel_for(n, [&](int i) {
-            int src = n - 1 - i;
-            int p = pl.ia[i];
-            for (int j = ia[src + 1] - 1; j >= ia[src]; j--, p++) {
-                pl.a[p] = a[j];
-                pl.ja[p] = n - 1 - ja[j];
+        parallel_chunks(n, 1 << 14, [&](int, int64_t lo, int64_t hi) {
+            for (int i = (int)lo; i < (int)hi; i++) {
+                const int src = n - 1 - i;
+                int p = pl.ia[i];
+                for (int j = ia[src + 1] - 1; j >= ia[src]; j--, p++) {
+                    pl.a[p] = a[j];
+                    pl.ja[p] = n - 1 - ja[j];
+                }
             }
         });
     }
@@ -372,14 +365,14 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
 
     // block-solve step streams
     std::vector<int> inblock_len(n);
-    for (int t = 0; t < P; t++) {
+    parallel_for(P, [&](int t) {
         const Part &pt = pl.parts[t];
         for (int r = 0; r < pt.k; r++) {
             int i = pt.s + r, c = 0;
             for (int j = LA[i]; j < LA[i + 1] - 1; j++) c += (LJ[j] >= pt.s);
             inblock_len[i] = c;
         }
-    }
+    });
     int G = opt.lanes_per_row > 0 ? opt.lanes_per_row
                                   : (opt.fp32 ? choose_lanes(inblock_len) : choose_block_lanes(inblock_len));
     if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16) return "lanes_per_row must be 1, 2, 4, 8 or 16";
@@ -387,8 +380,10 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     const bool fwd = opt.forward && G == 16;
     StepStream &ss = pl.steps;
     ss.G = G; ss.M = M; ss.R = 64 / G; ss.fwd = fwd;
-    pl.inv_diag.resize(n);
-    for (int i = 0; i < n; i++) pl.inv_diag[i] = 1.0 / LV[LA[i + 1] - 1];
+    resize_large(pl.inv_diag, n);
+    parallel_chunks(n, 1 << 16, [&](int, int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) pl.inv_diag[i] = 1.0 / LV[LA[i + 1] - 1];
+    });
     std::vector<std::vector<double>> pv(P), pl_late(P);
     std::vector<std::vector<uint16_t>> pc(P), pr(P);
     std::vector<int> pn(P, 0);
@@ -412,9 +407,9 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
         ss.max_steps = std::max(ss.max_steps, pn[t]);
     }
     ss.nsteps = ss.part_begin[P];
-    ss.val.resize((size_t)ss.nsteps * M * 64);
-    ss.col.resize((size_t)ss.nsteps * M * 64);
-    ss.row.resize((size_t)ss.nsteps * ss.R);
+    resize_large(ss.val, (size_t)ss.nsteps * M * 64);
+    resize_large(ss.col, (size_t)ss.nsteps * M * 64);
+    resize_large(ss.row, (size_t)ss.nsteps * ss.R);
     if (fwd) ss.late.resize((size_t)ss.nsteps * ss.R * ss.R);
     parallel_for(P, [&](int t) {
         size_t s0 = (size_t)ss.part_begin[t];

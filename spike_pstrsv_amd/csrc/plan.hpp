@@ -15,11 +15,79 @@
 //   spike W_p        D_p^-1 C_p[:, K_p] restricted to rows [f_p, g_p]
 #pragma once
 
+#include <sys/mman.h>
+
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace spike {
+
+// Host threads for preprocessing: the hardware threads, at most 32.
+inline int host_threads()
+{
+    const int nt = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(nt, 32));
+}
+
+// fn(i) for every i in [0, count), indices taken from a shared counter by up
+// to host_threads() threads.
+template <class F>
+void parallel_for(int count, F &&fn)
+{
+    int nt = host_threads();
+    if (count < 2 * nt) nt = std::max(1, std::min(nt, count));
+    if (nt == 1) {
+        for (int i = 0; i < count; i++) fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&]() {
+            for (;;) {
+                int i = next.fetch_add(1);
+                if (i >= count) break;
+                fn(i);
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
+// fn(c, lo, hi) over [0, n) cut into chunks of about grain indices (c is the
+// chunk number, in index order); returns the chunk count.
+template <class F>
+int parallel_chunks(int64_t n, int64_t grain, F &&fn)
+{
+    if (n <= 0) return 0;
+    const int64_t g = std::max<int64_t>(1, grain);
+    const int nc = (int)std::min<int64_t>((n + g - 1) / g, 1 << 20);
+    const int64_t step = (n + nc - 1) / nc;
+    parallel_for(nc, [&](int c) {
+        const int64_t lo = (int64_t)c * step, hi = std::min(n, lo + step);
+        if (lo < hi) fn(c, lo, hi);
+    });
+    return nc;
+}
+
+// v resized to n value-initialised elements; an empty v of at least 4 MiB is
+// first reserved and advised for transparent huge pages, which cuts the
+// first-touch page faults of large host arrays (a no-op where the kernel
+// offers none).
+template <class T>
+void resize_large(std::vector<T> &v, size_t n)
+{
+    if (v.empty() && n * sizeof(T) >= ((size_t)4 << 20)) {
+        v.reserve(n);
+        const uintptr_t b = ((uintptr_t)v.data() + 4095) & ~(uintptr_t)4095;
+        const uintptr_t e = ((uintptr_t)v.data() + n * sizeof(T)) & ~(uintptr_t)4095;
+        if (e > b) (void)madvise((void *)b, e - b, MADV_HUGEPAGE);
+    }
+    v.resize(n);
+}
 
 // Block solves laid out as fixed-shape "steps" executed by one 64-lane
 // wavefront per partition; both solve passes run the same steps.  A step
