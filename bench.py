@@ -51,6 +51,13 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def workload_rhs(n: int):
+    """The single right-hand side of the banded_1m / lap2d_256 lines."""
+    from spike_pstrsv_amd import gen
+
+    return gen.rhs(n, 1, seed=gen.SEED_BASE + 2)
+
+
 def inblock_nnz(m, P: int) -> int:
     import numpy as np
 
@@ -95,11 +102,37 @@ def host_cpu():
     return max(1, threads), nproc, model
 
 
-def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
-    """Times the oracle's Spike restatement (OpenMP, threads pinned close to
-    cores) on the host."""
+def cpu_baseline(workload_name: str, m, b, x_gpu, seconds: float):
+    """The CPU baseline of `workload_name`, measured in a child process (the
+    only one whose OpenMP threads are pinned: a binding set in this process
+    would pin its main thread to one core once torch loads the OpenMP
+    runtime, and every host thread it starts, the library's preprocessing
+    threads among them, would share that core).  Returns the child's record
+    and the GPU solution's error against the oracle's sequential
+    substitution."""
+    import subprocess
+
     import numpy as np
 
+    from oracle import pyoracle as orc
+
+    env = dict(os.environ)
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--workload", workload_name, "--cpu-seconds", str(seconds),
+           "--cpu-baseline-child"]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed ({r.returncode}): {r.stderr[-2000:]}")
+    cpu = json.loads(r.stdout.strip().splitlines()[-1])
+    ref = orc.ldsol(m.a, m.ia, m.ja, b)
+    rel = float(np.max(np.abs(x_gpu - ref)) / np.max(np.abs(ref)))
+    return cpu, rel
+
+
+def cpu_baseline_measure(m, P: int, b, seconds: float):
+    """Times the oracle's Spike restatement (OpenMP, threads pinned close to
+    cores) on the host; runs in the child process of cpu_baseline."""
     from oracle import pyoracle as orc
 
     threads, nproc, model = host_cpu()
@@ -117,7 +150,6 @@ def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
     t1 = time.perf_counter()
     orc.ldsol(m.a, m.ia, m.ja, b)
     t_seq = time.perf_counter() - t1
-    rel = float(np.max(np.abs(x_gpu - ref)) / np.max(np.abs(ref)))
     cpu = {
         "value": m.n * count / el,
         "unit": UNIT,
@@ -130,7 +162,7 @@ def cpu_baseline(m, P: int, b, x_gpu, seconds: float):
         "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")},
         "competitors": mkl_competitors(m, b, ref, threads, min(seconds / 2.0, 5.0)),
     }
-    return cpu, rel
+    return cpu
 
 
 MKL_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libmkl_rt.so.1")
@@ -210,6 +242,11 @@ def mkl_competitors(m, b, ref, threads: int, seconds: float):
         lambda x: lib.mkl_cspblas_dcsrtrsv(b"L", b"N", b"N", ctypes.byref(nn), ptr(a), ptr(ia), ptr(ja), ptr(bb),
                                            ptr(x)))
     return out
+
+
+def progress(msg: str):
+    """One line on stderr, flushed (long workloads report their phases)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
 def oracle_err(m, B, X, uplo="L"):
@@ -307,15 +344,18 @@ def bench_batched(args):
 
     from spike_pstrsv_amd import gen, native
 
+    progress("batched: generating 4096 matrices")
     specs, mats = gen.batched(4096)
     ms_in = [(mm.a, mm.ia, mm.ja, s["nrhs"]) for s, mm in zip(specs, mats)]
     stream = torch.cuda.current_stream()
     res = {}
     x64 = None
     for prec in precisions(args):
+        progress(f"batched: creating the {prec} batch (autotuned)")
         t0 = time.perf_counter()
         bt = native.Batch("L", ms_in, precision=prec)
         create_s = time.perf_counter() - t0
+        progress(f"batched: {prec} batch created in {create_s:.1f} s; timing solves")
         L = bt.length
         b = np.empty(L)
         for i, s in enumerate(specs):
@@ -341,6 +381,7 @@ def bench_batched(args):
         elif x64 is not None:
             worst = max(reldiff(x[bt.offsets[i]:bt.offsets[i + 1]], x64[bt.offsets[i]:bt.offsets[i + 1]])
                         for i in range(len(specs)))
+        progress(f"batched: {prec} {ms:.2f} ms per solve; checking every column against the oracle")
         err = batch_oracle_err(specs, mats, bt.offsets, b, x)
         st = bt.stats()
         res[prec] = {"ms_per_solve": ms, "rows_rhs_per_s": L / (ms * 1e-3), "rel_err_vs_oracle": err,
@@ -452,14 +493,15 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=None)
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale factors)")
     args = ap.parse_args()
-    # the CPU baseline's OpenMP threads stay on their cores (read when the
-    # OpenMP runtime starts, so before anything loads it)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    if args.cpu_baseline_child:
+        m, P, _ = workload(args.workload)
+        print(json.dumps(cpu_baseline_measure(m, P, workload_rhs(m.n), args.cpu_seconds)), flush=True)
+        return 0
     if args.copy_calibration:
         import torch
 
@@ -492,7 +534,7 @@ def main() -> int:
         return bench_serving(args)
     m, P, desc = workload(args.workload)
     n = m.n
-    b = gen.rhs(n, 1, seed=gen.SEED_BASE + 2)
+    b = workload_rhs(n)
     t_before = native.timers()
     t0 = time.perf_counter()
     fac = native.Factor("L", m.a, m.ia, m.ja, partitions=P)
@@ -631,7 +673,7 @@ def main() -> int:
             fv.close()
     out["variants"] = var
     if not args.no_cpu_baseline:
-        cpu, rel = cpu_baseline(m, P, b, x_gpu, args.cpu_seconds)
+        cpu, rel = cpu_baseline(args.workload, m, b, x_gpu, args.cpu_seconds)
         out["cpu_baseline"] = cpu
         out["rel_err_vs_sequential_fp64"] = rel
     else:

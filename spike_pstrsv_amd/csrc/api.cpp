@@ -547,7 +547,8 @@ int setup_sync_free(spike_factor_s *f)
 // slots (default 4; 0 turns it off; a value set there also drops the size
 // gate) whose reduced system times the slot count reaches 2^25 entries: with
 // several right-hand sides one gather per column replaces one per entry, and
-// small reduced systems keep the row sweeps.  Measured on the 4096-matrix
+// small reduced systems keep the row sweeps (a factor none of whose solves
+// can pass both gates builds no items).  Measured on the 4096-matrix
 // batch (512-row partitions, kernel time per solve): 8 slots 14.8 -> 10.9 ms,
 // 4 slots 5.7 -> 4.4 ms, 2 slots 2.7 -> 3.4 ms; one RHS on the 1M-row banded
 // matrix 47 -> 60 us.
@@ -566,52 +567,114 @@ int build_jacobi_items(spike_factor_s *f, const std::vector<int> &hrp, const std
     constexpr int kTarget = 2048, kColCap = 256;
     const int nred = (int)pl.dep_rows.size();
     const size_t rnnz = (size_t)hrp[nred];
-    std::vector<int> items, kcol, pos(nred, -1), cols, stamp(nred, -1);
+    // no solve of this factor can take the item path: skip the build
+    if (f->max_nr < min_slots || (!f->ji_force && (int64_t)rnnz * f->max_nr < ((int64_t)1 << 25))) return SPIKE_OK;
     std::vector<uint16_t> lci(std::max<size_t>(rnnz, 1), 0);
-    int kmax = 0;
-    for (const spike::Part &pt : pl.parts) {
+    // partitions on host threads; a run's distinct columns live in an
+    // open-addressing set (generation-stamped, so a new run clears it free)
+    struct PartItems {
+        std::vector<int> items, kcol;
+        int kmax = 0;
+        bool too_wide = false;
+    };
+    std::vector<PartItems> per(pl.P);
+    spike::parallel_for(pl.P, [&](int t) {
+        const spike::Part &pt = pl.parts[t];
+        if (pt.dcnt == 0) return;
+        thread_local std::vector<int> hkey, hval, hgen, cols;
+        thread_local int gen = 0;
+        auto slot_of = [&](int c) {
+            const size_t mask = hkey.size() - 1;
+            size_t h = ((uint32_t)c * 2654435761u) & mask;
+            while (hgen[h] == gen && hkey[h] != c) h = (h + 1) & mask;
+            return h;
+        };
+        auto reserve = [&](size_t need) {   // room for `need` keys at load <= 1/2
+            size_t cap = std::max<size_t>(hkey.size(), 1024);
+            while (cap < 2 * need) cap *= 2;
+            if (cap == hkey.size()) return;
+            hkey.assign(cap, -1);
+            hval.assign(cap, 0);
+            hgen.assign(cap, -1);
+            gen++;
+            for (int c : cols) {   // re-insert the current run's columns
+                const size_t h = slot_of(c);
+                hkey[h] = c;
+                hgen[h] = gen;
+            }
+        };
+        PartItems &out = per[t];
         for (int q0 = pt.dbeg, end = pt.dbeg + pt.dcnt; q0 < end;) {
             // extend the run while it stays within kTarget entries and
             // kColCap distinct columns (a single row may exceed either)
+            gen++;
+            cols.clear();
+            reserve(0);
             int q1 = q0, ncol = 0;
             while (q1 < end) {
+                reserve((size_t)ncol + (size_t)(hrp[q1 + 1] - hrp[q1]));
                 int fresh = 0;
-                for (int j = hrp[q1]; j < hrp[q1 + 1]; j++)
-                    if (stamp[hci[j]] != q0) {
-                        stamp[hci[j]] = q0;
+                for (int j = hrp[q1]; j < hrp[q1 + 1]; j++) {
+                    const size_t h = slot_of(hci[j]);
+                    if (hgen[h] != gen) {
+                        hkey[h] = hci[j];
+                        hgen[h] = gen;
+                        cols.push_back(hci[j]);
                         fresh++;
                     }
-                if (q1 > q0 && (hrp[q1 + 1] - hrp[q0] > kTarget || ncol + fresh > kColCap)) break;
+                }
+                if (q1 > q0 && (hrp[q1 + 1] - hrp[q0] > kTarget || ncol + fresh > kColCap)) {
+                    cols.resize(ncol);   // row q1 opens the next run
+                    break;
+                }
                 ncol += fresh;
                 q1++;
             }
-            cols.assign(hci.begin() + hrp[q0], hci.begin() + hrp[q1]);
-            std::sort(cols.begin(), cols.end());
-            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-            if (cols.size() > 65535) return SPIKE_OK;
-            const int kb = (int)kcol.size();
-            for (size_t i = 0; i < cols.size(); i++) {
-                pos[cols[i]] = (int)i;
-                kcol.push_back(cols[i]);
+            if (cols.size() > 65535) {
+                out.too_wide = true;
+                return;
             }
-            for (int j = hrp[q0]; j < hrp[q1]; j++) lci[j] = (uint16_t)pos[hci[j]];
-            kmax = std::max(kmax, (int)cols.size());
-            items.insert(items.end(), {q0, q1, kb, kb + (int)cols.size()});
+            std::sort(cols.begin(), cols.end());
+            const int kb = (int)out.kcol.size();
+            for (size_t i = 0; i < cols.size(); i++) {
+                hval[slot_of(cols[i])] = (int)i;
+                out.kcol.push_back(cols[i]);
+            }
+            for (int j = hrp[q0]; j < hrp[q1]; j++) lci[j] = (uint16_t)hval[slot_of(hci[j])];
+            out.kmax = std::max(out.kmax, (int)cols.size());
+            out.items.insert(out.items.end(), {q0, q1, kb, kb + (int)cols.size()});
             q0 = q1;
         }
+    });
+    std::vector<int> items, kcol;
+    int kmax = 0;
+    for (const PartItems &pi : per) {
+        if (pi.too_wide) return SPIKE_OK;
+        const int off = (int)kcol.size();
+        for (size_t i = 0; i < pi.items.size(); i += 4)
+            items.insert(items.end(), {pi.items[i], pi.items[i + 1], pi.items[i + 2] + off, pi.items[i + 3] + off});
+        kcol.insert(kcol.end(), pi.kcol.begin(), pi.kcol.end());
+        kmax = std::max(kmax, pi.kmax);
     }
     const int nit = (int)(items.size() / 4);
     if (nit == 0) return SPIKE_OK;
+    // per item: the largest need and which needs below 64 occur
+    std::vector<int> maxneed(nit, 0);
+    std::vector<uint64_t> needs(nit, 0);
+    spike::parallel_chunks(nit, 4096, [&](int, int64_t lo, int64_t hi) {
+        for (int64_t t = lo; t < hi; t++)
+            for (int q = items[4 * t]; q < items[4 * t + 1]; q++) {
+                const int nd = row_sweeps[q];
+                maxneed[t] = std::max(maxneed[t], nd);
+                if (nd < 64) needs[t] |= (uint64_t)1 << nd;
+            }
+    });
     std::vector<int> lists;
     f->ji_off.assign(f->sweeps + 1, 0);
     for (int m = 1; m <= f->sweeps; m++) {
         for (int t = 0; t < nit; t++) {
-            bool comp = false, copy = false;
-            for (int q = items[4 * t]; q < items[4 * t + 1]; q++) {
-                const int nd = row_sweeps[q];
-                comp = comp || nd >= m;
-                copy = copy || nd == m - 1 || (nd == 0 && m == 2);
-            }
+            const bool comp = maxneed[t] >= m;
+            const bool copy = ((needs[t] >> (m - 1)) & 1) || (m == 2 && (needs[t] & 1));
             if (comp) lists.push_back(t | (int)0x80000000);
             else if (copy) lists.push_back(t);
         }
@@ -1066,20 +1129,23 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         // sweep m computes rows that need >= m sweeps and copies rows that
         // finished at m-1 (rows finished at 0 are copied in sweeps 1 and 2),
         // so both buffers hold every finished row's value
-        std::vector<int> lists;
-        f->sweep_off.assign(f->sweeps + 1, 0);
-        for (int m = 1; m <= f->sweeps; m++) {
+        std::vector<std::vector<int>> per(f->sweeps + 1);
+        spike::parallel_for(f->sweeps, [&](int mi) {
+            const int m = mi + 1;
+            std::vector<int> &l = per[m];
             for (int q = 0; q < nred; q++) {
                 const int need = row_sweeps[q];
                 int code = -1;
                 if (need >= m) code = q;
                 else if (need == m - 1 || (need == 0 && m == 2)) code = q | (int)0x80000000;
                 if (code == -1) continue;
-                lists.push_back(code);
-                lists.push_back(hrp[q]);
-                lists.push_back(hrp[q + 1]);
-                lists.push_back(0);
+                l.insert(l.end(), {code, hrp[q], hrp[q + 1], 0});
             }
+        });
+        std::vector<int> lists;
+        f->sweep_off.assign(f->sweeps + 1, 0);
+        for (int m = 1; m <= f->sweeps; m++) {
+            lists.insert(lists.end(), per[m].begin(), per[m].end());
             f->sweep_off[m] = (int)(lists.size() / 4);
         }
         if ((rc = f->upload(&f->d_sweep_rows, lists))) return rc;

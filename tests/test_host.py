@@ -292,3 +292,22 @@ def test_bench_mkl_competitors_match_oracle():
     for name in ("mkl_sparse_d_trsv", "mkl_cspblas_dcsrtrsv"):
         assert out[name]["rel_err_vs_oracle"] <= 1e-12, (name, out[name])
         assert out[name]["solves"] >= 1
+
+
+def test_bench_cpu_baseline_runs_in_a_pinned_child():
+    """bench.py's CPU baseline leg runs in a child process (pinned OpenMP
+    threads there only) and reports the restatement and the MKL competitors;
+    the solution handed in is compared with the oracle."""
+    import os
+    import sys
+
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import bench
+
+    m, P, _ = bench.workload("lap2d_256")
+    b = bench.workload_rhs(m.n)
+    x = orc.ldsol(m.a, m.ia, m.ja, b)
+    cpu, rel = bench.cpu_baseline("lap2d_256", m, b, x, 0.2)
+    assert rel == 0.0
+    assert cpu["value"] > 0 and cpu["kind"] == "port" and "competitors" in cpu
+    assert cpu["omp"]["OMP_PROC_BIND"] == os.environ.get("OMP_PROC_BIND", "close")
