@@ -2181,6 +2181,9 @@ struct spike_batch_s {
     const double *gb = nullptr;
     double *gx = nullptr;
     int graph_nodes = 0;
+    // milliseconds of each group's solve run alone (measured before the
+    // first capture; they order the groups over the graph's branches)
+    std::vector<float> group_ms;
     // host-pointer staging
     double *d_sb = nullptr, *d_sx = nullptr;
 
@@ -2638,13 +2641,50 @@ int batch_capture(spike_batch_s *bt, const double *b, double *x)
         bt->events.push_back(e);
     }
     hipStream_t s0 = bt->streams[0];
+    const int G = (int)bt->groups.size();
+    if ((int)bt->group_ms.size() != G) {
+        // each group's solve alone, once (it writes the x this solve writes)
+        bt->group_ms.assign(G, 0.0f);
+        hipEvent_t t0, t1;
+        HIP_TRY(hipEventCreate(&t0));
+        HIP_TRY(hipEventCreate(&t1));
+        int rc = SPIKE_OK;
+        for (int g = 0; g < G && !rc; g++) {
+            spike_factor_s *f = bt->groups[g];
+            (void)hipEventRecord(t0, s0);
+            rc = solve_chunk(f, b, f->plan.n, x, f->plan.n, f->batch_slots, s0, nullptr, nullptr);
+            (void)hipEventRecord(t1, s0);
+            (void)hipEventSynchronize(t1);
+            (void)hipEventElapsedTime(&bt->group_ms[g], t0, t1);
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        if (rc) {
+            bt->group_ms.clear();
+            return rc;
+        }
+    }
+    // longest first, each to the least loaded branch: the long chains start
+    // together and the short groups fill the other branches behind them
+    std::vector<int> order(G), branch(G, 0);
+    for (int g = 0; g < G; g++) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return bt->group_ms[p] > bt->group_ms[q]; });
+    {
+        std::vector<double> load(B, 0.0);
+        for (int g : order) {
+            const int br = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            branch[g] = br;
+            load[br] += bt->group_ms[g];
+        }
+    }
     HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
     int rc = SPIKE_OK;
     hipError_t e = hipEventRecord(bt->events[0], s0);
     for (int i = 1; i < B && e == hipSuccess; i++) e = hipStreamWaitEvent(bt->streams[i], bt->events[0], 0);
-    for (size_t g = 0; g < bt->groups.size() && e == hipSuccess && !rc; g++) {
-        spike_factor_s *f = bt->groups[g];
-        rc = solve_chunk(f, b, f->plan.n, x, f->plan.n, f->batch_slots, bt->streams[g % B], nullptr, nullptr);
+    for (int k = 0; k < G && e == hipSuccess && !rc; k++) {
+        spike_factor_s *f = bt->groups[order[k]];
+        rc = solve_chunk(f, b, f->plan.n, x, f->plan.n, f->batch_slots, bt->streams[branch[order[k]]], nullptr,
+                         nullptr);
     }
     for (int i = 1; i < B && e == hipSuccess && !rc; i++) {
         e = hipEventRecord(bt->events[i], bt->streams[i]);
