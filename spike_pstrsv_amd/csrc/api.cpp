@@ -1033,7 +1033,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
         bool exact_end = false;
         // row chunks of about 2^16 entries on the host threads
         const int64_t grain = std::max<int64_t>(256, (int64_t)nred * 65536 / std::max<int64_t>(rnnz, 1));
-        std::vector<double> cmax((size_t)std::min<int64_t>((nred + grain - 1) / grain, 1 << 20));
+        std::vector<double> cmax((size_t)spike::chunk_count(nred, grain));
         for (int j = 1; j <= 64; j++) {
             std::vector<double> r2;
             spike::resize_large(r2, nred);

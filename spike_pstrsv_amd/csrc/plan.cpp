@@ -33,7 +33,7 @@ std::string validate_rows(char uplo, int n, const int *ia, const int *ja, int lo
 std::string validate(char uplo, int n, const int *ia, const int *ja)
 {
     if (ia[0] != 0) return "ia[0] must be 0";
-    std::vector<std::string> err((size_t)std::min<int64_t>((n + 65535) / 65536, 1 << 20));
+    std::vector<std::string> err((size_t)chunk_count(n, 65536));
     const int nc = parallel_chunks(n, 65536, [&](int c, int64_t lo, int64_t hi) {
         err[c] = validate_rows(uplo, n, ia, ja, (int)lo, (int)hi);
     });

@@ -57,14 +57,21 @@ void parallel_for(int count, F &&fn)
     for (auto &x : th) x.join();
 }
 
-// fn(c, lo, hi) over [0, n) cut into chunks of about grain indices (c is the
-// chunk number, in index order); returns the chunk count.
-template <class F>
-int parallel_chunks(int64_t n, int64_t grain, F &&fn)
+// Chunks parallel_chunks(n, grain, ...) cuts [0, n) into.
+inline int chunk_count(int64_t n, int64_t grain)
 {
     if (n <= 0) return 0;
     const int64_t g = std::max<int64_t>(1, grain);
-    const int nc = (int)std::min<int64_t>((n + g - 1) / g, 1 << 20);
+    return (int)std::min<int64_t>((n + g - 1) / g, 1 << 20);
+}
+
+// fn(c, lo, hi) over [0, n) cut into chunk_count(n, grain) chunks of about
+// grain indices (c is the chunk number, in index order); returns the count.
+template <class F>
+int parallel_chunks(int64_t n, int64_t grain, F &&fn)
+{
+    const int nc = chunk_count(n, grain);
+    if (nc == 0) return 0;
     const int64_t step = (n + nc - 1) / nc;
     parallel_for(nc, [&](int c) {
         const int64_t lo = (int64_t)c * step, hi = std::min(n, lo + step);
