@@ -29,16 +29,32 @@ std::string validate_rows(char uplo, int n, const int *ia, const int *ja, int lo
     return "";
 }
 
-// The first defect in row order (the message a row-by-row scan gives).
+// The first defect in row order (the message a row-by-row scan gives).  Row
+// pointers are checked first, so no row's entries are read through a
+// pointer that decreases or passes ia[n].
 std::string validate(char uplo, int n, const int *ia, const int *ja)
 {
     if (ia[0] != 0) return "ia[0] must be 0";
-    std::vector<std::string> err((size_t)chunk_count(n, 65536));
-    const int nc = parallel_chunks(n, 65536, [&](int c, int64_t lo, int64_t hi) {
+    const int nnz = ia[n];
+    std::vector<int> bad((size_t)chunk_count(n, 1 << 18), n);
+    parallel_chunks(n, 1 << 18, [&](int c, int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++)
+            if (ia[i + 1] <= ia[i] || ia[i + 1] > nnz) {
+                bad[c] = (int)i;
+                break;
+            }
+    });
+    const int rows_ok = *std::min_element(bad.begin(), bad.end());
+    std::vector<std::string> err((size_t)chunk_count(rows_ok, 65536));
+    const int nc = parallel_chunks(rows_ok, 65536, [&](int c, int64_t lo, int64_t hi) {
         err[c] = validate_rows(uplo, n, ia, ja, (int)lo, (int)hi);
     });
     for (int c = 0; c < nc; c++)
         if (!err[c].empty()) return err[c];
+    if (rows_ok < n) {
+        if (ia[rows_ok + 1] <= ia[rows_ok]) return "row " + std::to_string(rows_ok) + " is empty (diagonal missing)";
+        return "row pointer ia[" + std::to_string(rows_ok + 1) + "] exceeds ia[n]";
+    }
     return "";
 }
 

@@ -119,6 +119,26 @@ def test_invalid_csr_rejected(bad):
     assert e.value.code == 2
 
 
+def test_invalid_csr_first_defect_and_pointer_jump():
+    """Validation reports the first defect in row order, and a row pointer
+    past ia[n] is reported without reading entries through it."""
+    from spike_pstrsv_amd import native
+
+    m = gen.banded(100, bw=8, q=4, plr=0.0, seed=5)
+    a, ia, ja = m.a.copy(), m.ia.copy(), m.ja.copy()
+    j = ia[20]
+    ja[j], ja[j + 1] = ja[j + 1], ja[j]         # row 20 unsorted
+    ia[61:] -= ia[61] - ia[60]                  # row 60 empty
+    with pytest.raises(native.SpikeError) as e:
+        native.analyze("L", a, ia.copy(), ja, partitions=4)
+    assert "row 20 " in str(e.value)
+    ia2 = m.ia.copy()
+    ia2[50] = 10**9                             # row 49 ends past ia[n]
+    with pytest.raises(native.SpikeError) as e:
+        native.analyze("L", m.a, ia2, m.ja, partitions=4)
+    assert "ia[50] exceeds ia[n]" in str(e.value)
+
+
 def test_invalid_arguments():
     from spike_pstrsv_amd import native
 
