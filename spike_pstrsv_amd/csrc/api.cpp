@@ -2483,7 +2483,10 @@ int batch_calibrate(spike_batch_s *bt)
 int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
 {
     const int scale = bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 2 : 1;
-    static const int cand[4][3] = {{512, 1024, 2048}, {256, 512, 1024}, {256, 512, 1024}, {128, 256, 512}};
+    // Candidates bracket the sizes the 4096-matrix banded batch settles on
+    // (2048, 1024, 512 and 512 rows at 1-8 slots: each the middle one, the
+    // larger and smaller neighbours measured slower).
+    static const int cand[4][3] = {{1024, 2048, 4096}, {512, 1024, 2048}, {256, 512, 1024}, {256, 512, 1024}};
     // The sample holds about 4096 x the largest candidate's rows
     // (SPIKE_PSTRSV_TUNE_SAMPLE: a multiple of 1024 x, default 4), so that it
     // is bandwidth-bound like the full batch: with 1024 x the sample favoured
