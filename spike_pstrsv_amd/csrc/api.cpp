@@ -2482,11 +2482,15 @@ int batch_calibrate(spike_batch_s *bt)
 // wins.
 int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
 {
-    const int scale = bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 2 : 1;
-    // Candidates bracket the sizes the 4096-matrix banded batch settles on
-    // (2048, 1024, 512 and 512 rows at 1-8 slots: each the middle one, the
-    // larger and smaller neighbours measured slower).
-    static const int cand[4][3] = {{1024, 2048, 4096}, {512, 1024, 2048}, {256, 512, 1024}, {256, 512, 1024}};
+    // FP64 candidates bracket the sizes the 4096-matrix banded batch settles
+    // on (2048, 1024, 512 and 512 rows at 1-8 slots: each the middle one, the
+    // larger and smaller neighbours measured slower).  FP32 keeps its own
+    // range: allowed 2048 rows at 8 slots, it picked 1024 there (within the
+    // 10% margin on the sample), and the full batch then calibrated two
+    // refinement steps instead of one (51.9 against 34.9 ms per solve).
+    static const int cand64[4][3] = {{1024, 2048, 4096}, {512, 1024, 2048}, {256, 512, 1024}, {256, 512, 1024}};
+    static const int cand32[4][3] = {{1024, 2048, 4096}, {512, 1024, 2048}, {512, 1024, 2048}, {256, 512, 1024}};
+    const int(*cand)[3] = bt->opts.precision == SPIKE_PREC_FP32_REFINED ? cand32 : cand64;
     // The sample holds about 4096 x the largest candidate's rows
     // (SPIKE_PSTRSV_TUNE_SAMPLE: a multiple of 1024 x, default 4), so that it
     // is bandwidth-bound like the full batch: with 1024 x the sample favoured
@@ -2502,7 +2506,7 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
     HIP_TRY(hipEventCreate(&e1));
     int rc = SPIKE_OK;
     for (int li = 0; li < 4 && !rc; li++) {
-        best[li] = cand[li][1] * scale;
+        best[li] = cand[li][1];
         std::vector<int> members;
         int64_t rows = 0;
         for (int i = 0; i < bt->count; i++) {
@@ -2514,7 +2518,7 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
             }
         }
         if (members.empty()) continue;
-        const int64_t target = 1024LL * cand[li][2] * scale * sample_scale;
+        const int64_t target = 1024LL * cand[li][2] * sample_scale;
         const int64_t want = bt->opts.autotune_sample > 0 ? bt->opts.autotune_sample : INT64_MAX;
         const size_t step = rows > target ? (size_t)(rows / target) : 1;
         std::vector<spike_batch_matrix_t> sm;
@@ -2544,7 +2548,7 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
         (void)hipMemcpy(db, hb.data(), sizeof(double) * len, hipMemcpyHostToDevice);
         double rate_of[3] = {-1.0, -1.0, -1.0};
         for (int ci = 0; ci < 3; ci++) {
-            const int c = cand[li][ci] * scale;
+            const int c = cand[li][ci];
             int r4[4] = {c, c, c, c};
             tmp.release();
             if (batch_build(&tmp, sm.data(), sidx, r4, bt->opts.precision == SPIKE_PREC_FP32_REFINED ? 1 : 0)) {
@@ -2574,12 +2578,12 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
         const double top = std::max(rate_of[0], std::max(rate_of[1], rate_of[2]));
         if (tlog)
             std::fprintf(stderr, "autotune: %d slots, %zu sample matrices, rows per partition %d / %d / %d: %.3e %.3e %.3e rows*RHS/s\n",
-                         1 << li, sm.size(), cand[li][0] * scale, cand[li][1] * scale, cand[li][2] * scale,
+                         1 << li, sm.size(), cand[li][0], cand[li][1], cand[li][2],
                          rate_of[0], rate_of[1], rate_of[2]);
         double base = -1.0;
         for (int ci = 0; ci < 3; ci++)
             if (top > 0 && rate_of[ci] >= 0.90 * top) {
-                best[li] = cand[li][ci] * scale;
+                best[li] = cand[li][ci];
                 base = rate_of[ci];
             }
         // lanes per row (rows per wavefront step): half and twice each
