@@ -495,8 +495,8 @@ __device__ __forceinline__ void block_pass(const BlockArgs &A, const FusedCtx &F
 #pragma unroll
         for (int d = 0; d < D; d++) load_step(d, (unsigned)(d < nst ? d : nst - 1));
 
-    // Pass 1 with one RHS: the local rows of the first 64*UP dependents
-    // (written back in the epilogue) are loaded here, with the right-hand
+    // Pass 1 with one RHS: the rows of the first 64*UP dependents (written
+    // back in the epilogue) are loaded here, with the right-hand
     // sides, so the epilogue issues no loads for them.  (Several RHS load
     // them per trip in the epilogue: holding them across the step loop raised
     // the register count of those variants.)
@@ -504,10 +504,14 @@ __device__ __forceinline__ void block_pass(const BlockArgs &A, const FusedCtx &F
     constexpr bool dep_early = PASS == 1 && NR == 1;
     int ir0[UP];
 #pragma unroll
-    for (int u = 0; u < UP; u++) ir0[u] = 0;
+    for (int u = 0; u < UP; u++) ir0[u] = s;
+    // (global rows: the partition start is subtracted at their use in the
+    // epilogue; subtracting it here made the compiler wait for these loads,
+    // and with them for every ring load issued before, ahead of the
+    // right-hand sides' loads)
     if (dep_early && pt.dcnt > 0)
 #pragma unroll
-        for (int u = 0; u < UP; u++) ir0[u] = A.dep_rows[pt.dbeg + min(lane + 64 * u, pt.dcnt - 1)] - s;
+        for (int u = 0; u < UP; u++) ir0[u] = A.dep_rows[pt.dbeg + min(lane + 64 * u, pt.dcnt - 1)];
     // right-hand sides; pass 2 replaces them by b - C x_K on the coupling rows
     if (PASS == 1) {
         load_rhs<NR, T>(A, m, xs, s, k, roff, true);
@@ -619,7 +623,7 @@ epilogue:
         if (dep_early) {
             T v[UP];
 #pragma unroll
-            for (int u = 0; u < UP; u++) v[u] = xs[ir0[u]];
+            for (int u = 0; u < UP; u++) v[u] = xs[ir0[u] - s];
 #pragma unroll
             for (int u = 0; u < UP; u++)
                 if (lane + 64 * u < pt.dcnt) {
