@@ -257,6 +257,20 @@ struct spike_factor_s {
 
 namespace {
 
+// n timing events, all or none (on failure the created ones are destroyed).
+int create_events(hipEvent_t *ev, int n)
+{
+    for (int i = 0; i < n; i++) {
+        ev[i] = nullptr;
+        if (hipEventCreate(&ev[i]) != hipSuccess) {
+            for (int j = 0; j < i; j++) (void)hipEventDestroy(ev[j]);
+            (void)hipGetLastError();
+            return fail(SPIKE_ERR_DEVICE, "event creation failed");
+        }
+    }
+    return SPIKE_OK;
+}
+
 std::vector<float> to_f32(const std::vector<double> &v)
 {
     return std::vector<float>(v.begin(), v.end());
@@ -407,9 +421,9 @@ int tune_tiles(spike_factor_s *f)
     int rc;
     if ((rc = f->dalloc(&db, (size_t)n * 8)) || (rc = f->dalloc(&dx, (size_t)n * 8))) return rc;
     HIP_TRY(hipMemset(db, 0, sizeof(double) * (size_t)n * 8));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipEvent_t ee[2];
+    if ((rc = create_events(ee, 2))) return rc;
+    hipEvent_t e0 = ee[0], e1 = ee[1];
     for (int li = 1; li < 4 && (1 << li) <= f->max_nr && !rc; li++) {
         const int c = 1 << li;
         if ((rc = prepare_fused(f, c))) break;
@@ -2044,7 +2058,7 @@ int factor_solve(spike_factor_s *f, int nrhs, const double *b, int ldb, double *
     }
     hipEvent_t ev[5];
     if (timed)
-        for (int i = 0; i < 5; i++) HIP_TRY(hipEventCreate(&ev[i]));
+        if (int erc = create_events(ev, 5)) return erc;
     for (int j = 0; j < nrhs; j += f->max_nr) {
         const int nr = std::min(f->max_nr, nrhs - j);
         double ms[4] = {0.0, 0.0, 0.0, 0.0};
@@ -2501,9 +2515,9 @@ int autotune_rpp(spike_batch_s *bt, const spike_batch_matrix_t *m, int *best)
     int sample_scale = 4;
     if (const char *e = std::getenv("SPIKE_PSTRSV_TUNE_SAMPLE")) sample_scale = std::max(1, std::atoi(e));
     const bool tlog = std::getenv("SPIKE_PSTRSV_TUNE_LOG") && std::atoi(std::getenv("SPIKE_PSTRSV_TUNE_LOG")) == 1;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipEvent_t ee[2];
+    if (int erc = create_events(ee, 2)) return erc;
+    hipEvent_t e0 = ee[0], e1 = ee[1];
     int rc = SPIKE_OK;
     for (int li = 0; li < 4 && !rc; li++) {
         best[li] = cand[li][1];
@@ -2652,9 +2666,12 @@ int batch_capture(spike_batch_s *bt, const double *b, double *x)
     if ((int)bt->group_ms.size() != G) {
         // each group's solve alone, once (it writes the x this solve writes)
         bt->group_ms.assign(G, 0.0f);
-        hipEvent_t t0, t1;
-        HIP_TRY(hipEventCreate(&t0));
-        HIP_TRY(hipEventCreate(&t1));
+        hipEvent_t tt[2];
+        if (int erc = create_events(tt, 2)) {
+            bt->group_ms.clear();
+            return erc;
+        }
+        hipEvent_t t0 = tt[0], t1 = tt[1];
         int rc = SPIKE_OK;
         for (int g = 0; g < G && !rc; g++) {
             spike_factor_s *f = bt->groups[g];
@@ -2720,9 +2737,9 @@ int batch_capture(spike_batch_s *bt, const double *b, double *x)
 // solves, then the fastest of three rounds of three.
 int time_factor(spike_factor_s *f, const double *db, double *dx, double *ms_out)
 {
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipEvent_t ee[2];
+    if (int erc = create_events(ee, 2)) return erc;
+    hipEvent_t e0 = ee[0], e1 = ee[1];
     const int n = f->plan.n;
     int rc = SPIKE_OK;
     for (int r = 0; r < 2 && !rc; r++) rc = factor_solve(f, 1, db, n, dx, n, nullptr, false);
