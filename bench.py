@@ -136,12 +136,16 @@ def cpu_baseline_measure(m, P: int, b, seconds: float):
     from oracle import pyoracle as orc
 
     threads, nproc, model = host_cpu()
+    t0 = time.perf_counter()
     sp = orc.Spike("L", m.a, m.ia, m.ja, P, tau=0.0, nthreads=threads)
+    preproc_s = time.perf_counter() - t0
     x = sp.solve(b)
     x = sp.solve(b)
-    count, t0 = 0, time.perf_counter()
+    count, per, t0 = 0, [], time.perf_counter()
     while True:
+        ts = time.perf_counter()
         sp.solve(b, x)
+        per.append(time.perf_counter() - ts)
         count += 1
         el = time.perf_counter() - t0
         if el >= seconds or count >= 5000:
@@ -160,9 +164,19 @@ def cpu_baseline_measure(m, P: int, b, seconds: float):
         "nproc": nproc,
         "cpu_model": model,
         "omp": {"OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")},
+        "preproc_s": preproc_s,
+        "solve_ms": solve_stats(per),
         "competitors": mkl_competitors(m, b, ref, threads, min(seconds / 2.0, 5.0)),
     }
     return cpu
+
+
+def solve_stats(per):
+    """min / mean / p50 / p99 of per-solve seconds, in milliseconds (the
+    statistics test/runtime_analysis.c reports for its timed solves)."""
+    v = sorted(per)
+    return {"min": v[0] * 1e3, "mean": statistics.fmean(v) * 1e3, "p50": statistics.median(v) * 1e3,
+            "p99": v[min(len(v) - 1, int(0.99 * len(v)))] * 1e3, "solves": len(v)}
 
 
 MKL_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so", "libmkl_rt.so.1")
@@ -203,15 +217,17 @@ def mkl_competitors(m, b, ref, threads: int, seconds: float):
     def sample(call):
         x = np.zeros(n)
         call(x)                                           # warm-up
-        count, t0 = 0, time.perf_counter()
+        count, per, t0 = 0, [], time.perf_counter()
         while True:
+            ts = time.perf_counter()
             call(x)
+            per.append(time.perf_counter() - ts)
             count += 1
             el = time.perf_counter() - t0
             if el >= seconds or count >= 5000:
                 break
         return {"value": n * count / el, "unit": UNIT, "solves": count, "seconds": round(el, 3),
-                "rel_err_vs_oracle": reldiff(x, ref)}
+                "solve_ms": solve_stats(per), "rel_err_vs_oracle": reldiff(x, ref)}
 
     out = {"available": True, "library": path, "threads": int(lib.MKL_Get_Max_Threads()),
            "threading_layer": os.environ.get("MKL_THREADING_LAYER")}
