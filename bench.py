@@ -136,6 +136,9 @@ def cpu_baseline_measure(m, P: int, b, seconds: float):
     from oracle import pyoracle as orc
 
     threads, nproc, model = host_cpu()
+    # at most one thread per partition (the reference runs one partition per
+    # OpenMP thread, src/spike_pstrsv.c:168-170): 8 threads for the 2D Laplacian's P = 8
+    threads = min(threads, P)
     t0 = time.perf_counter()
     sp = orc.Spike("L", m.a, m.ia, m.ja, P, tau=0.0, nthreads=threads)
     preproc_s = time.perf_counter() - t0
