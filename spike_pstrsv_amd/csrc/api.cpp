@@ -85,6 +85,9 @@ struct spike_factor_s {
     std::vector<void *> allocs;
     std::vector<size_t> alloc_bytes;   // size of allocs[i]
     int64_t device_bytes = 0;
+    // SPIKE_PSTRSV_FAIL_ALLOC=<k> (tests): the factor's k-th device
+    // allocation fails as if the device were out of memory
+    int fail_alloc_at = 0, nalloc = 0;
 
     DevPart *d_parts = nullptr;
     int *d_dep_rows = nullptr;
@@ -193,6 +196,9 @@ struct spike_factor_s {
     {
         *p = nullptr;
         if (count == 0) count = 1;
+        if (fail_alloc_at > 0 && ++nalloc == fail_alloc_at)
+            return fail(SPIKE_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                             " bytes failed (simulated by SPIKE_PSTRSV_FAIL_ALLOC)");
         hipError_t e = hipMalloc((void **)p, count * sizeof(T));
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -792,6 +798,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
     const double t0 = now_s();
     int rc;
     HIP_TRY(hipGetDevice(&f->device));
+    if (const char *e = std::getenv("SPIKE_PSTRSV_FAIL_ALLOC")) f->fail_alloc_at = std::max(0, std::atoi(e));
     spike::PlanOptions po;
     po.kmax_limit = kmax_limit_for_device();
     int P = f->opts.partitions;

@@ -374,3 +374,37 @@ def test_autotuned_partitions_and_lanes(native):
                                    min(16, 2 * s0["lanes_per_row"]))
     x = f2.solve(b)
     assert relerr(x, orc.ldsol(m.a, m.ia, m.ja, b)) <= TOL
+
+
+def test_allocation_failure_is_reported_and_releases_memory(native, monkeypatch):
+    """A device allocation that fails during factor creation (forced from the
+    host: SPIKE_PSTRSV_FAIL_ALLOC=k fails the factor's k-th) returns
+    SPIKE_ERR_NOMEM, frees what the factor had allocated, and leaves the
+    library usable."""
+    import torch
+
+    m = gen.banded(1 << 16, 64, 16, 0.02, gen.SEED_BASE + 90)
+    native.Factor("L", m.a, m.ia, m.ja, partitions=64).close()
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    failures = 0
+    for k in range(1, 400, 3):
+        monkeypatch.setenv("SPIKE_PSTRSV_FAIL_ALLOC", str(k))
+        try:
+            f = native.Factor("L", m.a, m.ia, m.ja, partitions=64)
+        except native.SpikeError as e:
+            assert e.code == 4, (k, str(e))
+            failures += 1
+            continue
+        f.close()
+        break                                   # k is past the last allocation
+    monkeypatch.delenv("SPIKE_PSTRSV_FAIL_ALLOC")
+    print("forced allocation failures:", failures)
+    assert failures >= 5
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    assert free0 - free1 < (64 << 20), (free0, free1)
+    f = native.Factor("L", m.a, m.ia, m.ja, partitions=64)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 91).ravel()
+    assert relerr(np.asarray(f.solve(b)).ravel(), orc.ldsol(m.a, m.ia, m.ja, b)) <= TOL
+    f.close()

@@ -331,3 +331,21 @@ def test_bench_cpu_baseline_runs_in_a_pinned_child():
     assert rel == 0.0
     assert cpu["value"] > 0 and cpu["kind"] == "port" and "competitors" in cpu
     assert cpu["omp"]["OMP_PROC_BIND"] == os.environ.get("OMP_PROC_BIND", "close")
+
+
+def test_legacy_solve_before_preprocess_leaves_x(capfd):
+    """The legacy solve without a preprocessed matrix prints one line and
+    leaves x untouched (the reference would read uninitialised globals)."""
+    import ctypes as C
+
+    from spike_pstrsv_amd import native
+
+    if native.lib.spike_pstrsv_current_factor():
+        pytest.skip("a legacy factor is current in this process")
+    n = 16
+    b = np.ones(n)
+    x = np.full(n, 7.0)
+    native.lib.spike_pstrsv_solve(b"L", n, C.c_void_p(b.ctypes.data), C.c_void_p(x.ctypes.data), 1)
+    err = capfd.readouterr().err
+    assert "no preprocessed matrix" in err
+    assert np.all(x == 7.0) and np.all(b == 1.0)
