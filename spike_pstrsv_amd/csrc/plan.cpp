@@ -281,6 +281,27 @@ std::string build_plan(char uplo, int n, const double *a, const int *ia, const i
     const int *LJ = pl.ja.data();
     const double *LV = pl.a.data();
 
+    // a zero or non-finite diagonal would be divided by: rejected, naming the
+    // first such row in the caller's order (the last in lower-form order for
+    // 'U')
+    {
+        const int nc = chunk_count(n, 1 << 16);
+        std::vector<int> first(nc, n), last(nc, -1);
+        parallel_chunks(n, 1 << 16, [&](int c, int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; i++) {
+                const double d = LV[LA[i + 1] - 1];
+                if (d == 0.0 || !std::isfinite(d)) {
+                    first[c] = std::min(first[c], (int)i);
+                    last[c] = (int)i;
+                }
+            }
+        });
+        const int lo = *std::min_element(first.begin(), first.end());
+        const int hi = *std::max_element(last.begin(), last.end());
+        if (lo < n)
+            return "row " + std::to_string(pl.flip ? n - 1 - hi : lo) + ": the diagonal is zero or not finite";
+    }
+
     // partitions (matrix_partitioning.c:18-33): equal blocks, remainder last;
     // or the caller's boundaries (the reference's IS_MATRIX_PARTITIONED path,
     // matrix_partitioning.c:23, :34-35)

@@ -95,7 +95,7 @@ def test_block_lanes_choice(case, precision, lanes):
     assert st["steps_pass1"] >= int(np.max((lens + lanes - 1) // lanes))
 
 
-@pytest.mark.parametrize("bad", ["unsorted", "nodiag", "emptyrow", "ia0", "range"])
+@pytest.mark.parametrize("bad", ["unsorted", "nodiag", "emptyrow", "ia0", "range", "zerodiag", "nandiag"])
 def test_invalid_csr_rejected(bad):
     from spike_pstrsv_amd import native
 
@@ -114,9 +114,13 @@ def test_invalid_csr_rejected(bad):
         ia = ia + 1
     elif bad == "range":
         ja[3] = 10_000
+    elif bad in ("zerodiag", "nandiag"):
+        a[ia[51] - 1] = 0.0 if bad == "zerodiag" else np.nan   # row 50's diagonal (last in 'L')
     with pytest.raises(native.SpikeError) as e:
         native.analyze("L", a, ia, ja, partitions=4)
     assert e.value.code == 2
+    if bad in ("zerodiag", "nandiag"):
+        assert "row 50:" in str(e.value)
 
 
 def test_invalid_csr_first_defect_and_pointer_jump():
@@ -349,3 +353,17 @@ def test_legacy_solve_before_preprocess_leaves_x(capfd):
     err = capfd.readouterr().err
     assert "no preprocessed matrix" in err
     assert np.all(x == 7.0) and np.all(b == 1.0)
+
+
+def test_zero_diagonal_named_in_callers_order_for_upper():
+    """A zero diagonal in an upper triangle is reported at its own row (the
+    first such row of the caller's matrix), not at its lower-form index."""
+    from spike_pstrsv_amd import native
+
+    u = gen.to_upper(gen.banded(100, bw=8, q=4, plr=0.0, seed=5))
+    a = u.a.copy()
+    for r in (30, 70):
+        a[u.ia[r]] = 0.0                        # the diagonal is first in 'U'
+    with pytest.raises(native.SpikeError) as e:
+        native.analyze("U", a, u.ia, u.ja, partitions=4)
+    assert e.value.code == 2 and "row 30:" in str(e.value)
