@@ -9,7 +9,8 @@
  * Matrices are zero-based 3-array CSR (ia[0] == 0, columns ascending, no empty
  * rows), with the diagonal stored as the first entry of each row for
  * uplo 'U'/'u' and as the last entry for 'L'/'l' (reference
- * src/spike_pstrsv_ops.c:1186 and :1277).
+ * src/spike_pstrsv_ops.c:1186 and :1277).  A zero or non-finite diagonal
+ * value is rejected (SPIKE_ERR_MATRIX); the reference divides by it.
  *
  * Differences in behaviour from the reference, all compatible with its callers:
  *   - 'L' is solved (the reference leaves x untouched for 'L');
