@@ -372,7 +372,7 @@ def bench_batched(args):
     for prec in precisions(args):
         progress(f"batched: creating the {prec} batch (autotuned)")
         t0 = time.perf_counter()
-        bt = native.Batch("L", ms_in, precision=prec)
+        bt = native.Batch("L", ms_in, precision=prec, use_graph=not args.no_graph)
         create_s = time.perf_counter() - t0
         progress(f"batched: {prec} batch created in {create_s:.1f} s; timing solves")
         L = bt.length
@@ -417,7 +417,7 @@ def bench_batched(args):
     b_alg = 12 * nnz + 4 * (rows + len(specs)) + 16 * rows_rhs
     out = base_line(args, best["rows_rhs_per_s"], best["ms_per_solve"], "f64",
                     {"workload": "batched_4096", "matrices": len(specs), "rows": rows, "nnz": nnz,
-                     "rows_rhs": rows_rhs})
+                     "rows_rhs": rows_rhs, "graph": not args.no_graph})
     out["precisions"] = res
     out["solve_roofline_frac"] = b_alg / (best["ms_per_solve"] * 1e-3) / (HBM_PEAK_GBS * 1e9)
     # the target's ratio: batched FP32 + refinement over the plain FP64 path
@@ -512,6 +512,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=None)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched: launch the groups eagerly on one stream instead of the captured graph")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy three times (reference kernel for the "
