@@ -379,14 +379,23 @@ void choose_rhs_slots(spike_factor_s *f)
         f->fuse_at[li] = false;
     }
     if (f->batch_slots > 0) {
-        // a batch group: one tile = its slot count
         const int li = __builtin_ctz((unsigned)f->batch_slots);
         if (x_per * f->batch_slots > lim) {
             f->max_nr = 0;
             return;
         }
-        f->tile_at[li] = f->batch_slots;
-        f->fuse_at[li] = fuse_ok(f, f->batch_slots, f->batch_slots);
+        // a batch group: waves of at most 4 of its RHS slots (8-slot groups
+        // run two waves per partition, which halves the LDS x segment of a
+        // wave; on the 4096-matrix batch 19.7 ms per solve against 20.2 with
+        // 8-RHS waves and 25.2 with 2-RHS waves); SPIKE_PSTRSV_BATCH_TILE=<t>
+        // (1, 2, 4 or 8) sets another bound
+        int t = std::min(f->batch_slots, 4);
+        if (const char *e = std::getenv("SPIKE_PSTRSV_BATCH_TILE")) {
+            const int w = std::atoi(e);
+            if (w == 1 || w == 2 || w == 4 || w == 8) t = std::min(f->batch_slots, w);
+        }
+        f->tile_at[li] = t;
+        f->fuse_at[li] = fuse_ok(f, f->batch_slots, t);
         f->max_nr = f->batch_slots;
         return;
     }
