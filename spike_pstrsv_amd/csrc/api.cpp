@@ -387,7 +387,8 @@ void choose_rhs_slots(spike_factor_s *f)
         // a batch group: waves of at most 4 of its RHS slots (8-slot groups
         // run two waves per partition, which halves the LDS x segment of a
         // wave; on the 4096-matrix batch 19.7 ms per solve against 20.2 with
-        // 8-RHS waves and 25.2 with 2-RHS waves); SPIKE_PSTRSV_BATCH_TILE=<t>
+        // 8-RHS waves, 25.2 with 2-RHS waves, and 20.1 with 2-RHS waves for
+        // the 4-slot groups only); SPIKE_PSTRSV_BATCH_TILE=<t>
         // (1, 2, 4 or 8) sets another bound
         int t = std::min(f->batch_slots, 4);
         if (const char *e = std::getenv("SPIKE_PSTRSV_BATCH_TILE")) {
