@@ -727,7 +727,8 @@ int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
     Plan &pl = f->plan;
     const int n = pl.n;
     int rc;
-    // Row records and strictly lower entries in level order.  Every level
+    // Row records {row, first entry, end entry, the row's first entry in the
+    // lower-form CSR} and strictly lower entries in level order.  Every level
     // starts at a multiple of 64 positions (padding records have row -1), so
     // no wavefront holds rows of two levels: in the sync-free solve a row may
     // wait for another wavefront's rows, never for its own wavefront's.
@@ -744,7 +745,7 @@ int build_levels(spike_factor_s *f, const std::vector<int> &rows, double t0)
                 lja.push_back(pl.ja[e]);
                 lval.push_back(pl.a[e]);
             }
-            rec.insert(rec.end(), {row, eb, (int)lja.size(), 0});
+            rec.insert(rec.end(), {row, eb, (int)lja.size(), pl.ia[row]});
         }
         while ((rec.size() / 4) % 64) rec.insert(rec.end(), {-1, 0, 0, 0});
         padded_ptr.push_back((int)(rec.size() / 4));
@@ -1266,7 +1267,7 @@ int factor_build(spike_factor_s *f, char uplo, int n, const double *a, const int
 // File layout: magic, a block of scalars, then tagged records
 // {tag (4 chars), byte count (8), bytes}.  Host vectors and device arrays are
 // stored by content; workspaces only by size (re-allocated on load).
-constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '7'};
+constexpr char kMagic[8] = {'S', 'P', 'K', 'F', 'A', 'C', '0', '8'};
 
 struct FactorScalars {
     spike_options_t opts;
@@ -1619,6 +1620,7 @@ spike::LevelArgs level_il_args(const spike_factor_s *f)
     la.invd = f->d_inv_diag;
     la.lval32 = f->d_lval32;
     la.invd32 = f->d_invd32;
+    la.a64 = f->d_a;
     la.fp32 = f->fp32 ? 1 : 0;
     la.spin_limit = f->spin_limit;
     la.fail_word = f->d_fail;
@@ -1828,9 +1830,12 @@ int residual(spike_factor_s *f, const double *b, int ldb, const double *x, int l
 // then refine_steps times r = b - A x (FP64), x += M^-1 r.  The residual's
 // device time is added to ph[2].
 // FP32 level solve of 2..32 right-hand sides with one refinement step, kept
-// interleaved between its launches: x0 = M^-1 b into d_xi, r = b - A x0
-// (FP64) into d_r, d = M^-1 r into d_bi (b is not read after the residual),
-// and x = x0 + d in one de-interleave.
+// interleaved between its launches.  Sync-free: one launch computes x0, the
+// FP64 residual and the correction row by row (packed into d_xi), and one
+// de-interleave writes x0 + d.  One launch per level: x0 = M^-1 b into
+// d_xi, r = b - A x0 (FP64) into d_r, d = M^-1 r into d_bi (b is not read
+// after the residual), and x = x0 + d in one de-interleave.  Both give the
+// same bits.
 int solve_levels_refined1(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s)
 {
     const int n = f->plan.n;
@@ -1840,6 +1845,11 @@ int solve_levels_refined1(spike_factor_s *f, const double *b, int ldb, double *x
     const spike::LevelArgs la = level_il_args(f);
     const int flip = f->plan.flip ? 1 : 0;
     HIP_TRY(spike::launch_interleave(n, nr, nrp, flip, b, ldb, f->d_bi, s));
+    if (f->sync_free) {
+        HIP_TRY(spike::launch_sync_free_refined1(f->lev_ptr.back(), n, nrp, la, f->d_bi, f->d_xi, f->d_abort, s));
+        HIP_TRY(spike::launch_deinterleave_pairs(n, nr, nrp, flip, f->d_xi, x, ldx, s));
+        return SPIKE_OK;
+    }
     HIP_TRY(level_il_solve(f, la, nrp, f->d_bi, f->d_xi, s));
     HIP_TRY(spike::launch_residual_il(n, nrp, f->d_ia, f->d_ja, f->d_a, f->d_bi, f->d_xi, f->d_r, s));
     HIP_TRY(level_il_solve(f, la, nrp, f->d_r, f->d_bi, s));

@@ -1554,6 +1554,90 @@ __global__ __launch_bounds__(256) void k_sf_il(int cnt, const int4 *__restrict__
     store_agent(&xi[(size_t)row * NRP + r], ok ? (double)(((T)bv - acc) * d) : __builtin_nan(""));
 }
 
+// FP32 level solve with one FP64 refinement step in one sync-free launch, on
+// interleaved right-hand sides (NRP lanes per row as in k_sf_il).  Per row
+// and right-hand side, in level order:
+//   x0 = (b - sum_e l_e x0[c_e]) / d      FP32 values (k_sf_il<NRP, float>)
+//   r  = b - sum_e a_e x0[c_e] - a_d x0    FP64, CSR order with the diagonal
+//                                          last (k_residual_il)
+//   dx = (r - sum_e l_e dx[c_e]) / d       FP32 values (k_sf_il<NRP, float>)
+// with the same operations in the same order as those three launches, so
+// x = x0 + dx has the same bits as the launch sequence.  Both solves walk the
+// same levels, so one wait per operand serves both: x0 and dx are FP32
+// numbers and are published together as one 64-bit word, x0 in the high
+// half and dx in the low half (a poll sees both or neither).  No published
+// word equals kNotReady: its high half is a NaN and its low half a finite
+// number, while a NaN x0 makes r, and so dx, NaN.  rec.w is the row's first
+// entry in the lower-form CSR (a64, diagonal last), whose strictly lower
+// entries are the row's level-ordered entries in the same order.
+__device__ __forceinline__ double pack_pair(float hi, float lo)
+{
+    return __longlong_as_double((long long)(((unsigned long long)__float_as_uint(hi) << 32) | __float_as_uint(lo)));
+}
+__device__ __forceinline__ float pair_hi(double v)
+{
+    return __uint_as_float((unsigned)((unsigned long long)__double_as_longlong(v) >> 32));
+}
+__device__ __forceinline__ float pair_lo(double v)
+{
+    return __uint_as_float((unsigned)(unsigned long long)__double_as_longlong(v));
+}
+
+template <int NRP>
+__global__ __launch_bounds__(256) void k_sf_il_ref1(int cnt, const int4 *__restrict__ rec, const int *__restrict__ lja,
+                                                    const float *__restrict__ lval, const float *__restrict__ invd,
+                                                    const double *__restrict__ a64, const double *__restrict__ bi,
+                                                    double *pub, int spin_limit, int *abort_word, int *fail_word)
+{
+    constexpr int U = 4;
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int q = gid / NRP, r = gid % NRP;
+    if (q >= cnt) return;
+    const int4 rc = rec[q];
+    asm volatile("" ::"v"(rc.y), "v"(rc.z), "v"(rc.w));   // one record load, not two around the branch
+    if (rc.x < 0) return;   // padding
+    const int row = rc.x;
+    const double bv = bi[(size_t)row * NRP + r];
+    const float d = invd[row];
+    const int abase = rc.w - rc.y;   // a64 index of level-ordered entry e: abase + e
+    const double ad = a64[abase + rc.z];   // the diagonal, after the strictly lower entries
+    float acc0 = 0.f, acc1 = 0.f;
+    double acc64 = 0.0;
+    bool ok = true;
+    for (int e = rc.y; e < rc.z && ok; e += U) {
+        size_t idx[U];
+        bool in[U];
+        float w[U];
+        double w64[U], pv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // unconditional loads at a clamped index; a masked entry polls
+            // the last entry again and is not used
+            in[u] = e + u < rc.z;
+            const int ee = in[u] ? e + u : rc.z - 1;
+            idx[u] = (size_t)lja[ee] * NRP + r;
+            const float wl = lval[ee];
+            const double wl64 = a64[abase + ee];
+            w[u] = in[u] ? wl : 0.f;
+            w64[u] = in[u] ? wl64 : 0.0;
+        }
+        ok = poll_values<U>(pub, idx, in, pv, spin_limit, abort_word, fail_word);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u]) {
+                const float x0c = pair_hi(pv[u]);
+                acc0 = fma(w[u], x0c, acc0);
+                acc64 = fma(w64[u], (double)x0c, acc64);
+                acc1 = fma(w[u], pair_lo(pv[u]), acc1);
+            }
+    }
+    const float x0 = ((float)bv - acc0) * d;
+    const double res = bv - fma(ad, (double)x0, acc64);
+    const float dx = ((float)res - acc1) * d;
+    const float qnan = __builtin_nanf("");
+    store_agent(&pub[(size_t)row * NRP + r], ok ? pack_pair(x0, dx) : pack_pair(qnan, qnan));
+}
+
 // v[0 .. len) = kNotReady; clears the abort word.
 __global__ __launch_bounds__(256) void k_not_ready(size_t len, double *v, int *abort_word)
 {
@@ -1597,9 +1681,11 @@ __global__ __launch_bounds__(256) void k_interleave(int n, int nr, int lgp, int 
     }
 }
 // x (caller layout) = v, or x += v with accum, for the first nr columns;
-// with v2, v + v2 takes the place of v.
+// with v2, v + v2 takes the place of v; with pairs, v holds k_sf_il_ref1's
+// packed (x0, dx) words and x0 + dx (in FP64) takes the place of v.
 __global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int lgp, int flip, const double *__restrict__ v,
-                                                      const double *__restrict__ v2, double *x, int ld, int accum)
+                                                      const double *__restrict__ v2, double *x, int ld, int accum,
+                                                      int pairs)
 {
     __shared__ double tile[kTile + 32];
     const int nrp = 1 << lgp, lgr = 11 - lgp, TR = 1 << lgr;
@@ -1611,6 +1697,7 @@ __global__ __launch_bounds__(256) void k_deinterleave(int n, int nr, int lgp, in
         const size_t idx = ((size_t)i0 << lgp) + g;
         const bool ok = i0 + (g >> lgp) < n;
         w[u] = ok ? v[idx] : 0.0;
+        if (pairs) w[u] = (double)pair_hi(w[u]) + (double)pair_lo(w[u]);
         if (v2 != nullptr && ok) w[u] += v2[idx];
     }
 #pragma unroll
@@ -2382,6 +2469,35 @@ hipError_t launch_sync_free(int npos, int n, int nrp, const LevelArgs &a, const 
     }
 }
 
+template <int NRP>
+static hipError_t sf_il_ref1_n(int npos, const LevelArgs &a, const double *bi, double *pub, int *abort_word,
+                               hipStream_t s)
+{
+    const dim3 grid((unsigned)(((size_t)npos * NRP + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_sf_il_ref1<NRP>), grid, block, 0, s, npos, (const int4 *)a.rec, a.lja, a.lval32, a.invd32,
+                       a.a64, bi, pub, a.spin_limit, abort_word, a.fail_word);
+    return hipGetLastError();
+}
+
+hipError_t launch_sync_free_refined1(int npos, int n, int nrp, const LevelArgs &a, const double *bi, double *pub,
+                                     int *abort_word, hipStream_t s)
+{
+    if (!a.fp32 || a.a64 == nullptr || a.lval32 == nullptr || a.invd32 == nullptr) return hipErrorInvalidValue;
+    if (npos <= 0 || n <= 0) return hipSuccess;
+    const size_t len = (size_t)n * nrp;
+    hipLaunchKernelGGL(k_not_ready, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, pub, abort_word);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (nrp) {
+    case 2: return sf_il_ref1_n<2>(npos, a, bi, pub, abort_word, s);
+    case 4: return sf_il_ref1_n<4>(npos, a, bi, pub, abort_word, s);
+    case 8: return sf_il_ref1_n<8>(npos, a, bi, pub, abort_word, s);
+    case 16: return sf_il_ref1_n<16>(npos, a, bi, pub, abort_word, s);
+    case 32: return sf_il_ref1_n<32>(npos, a, bi, pub, abort_word, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 namespace {
 // log2 of an interleave width (a power of two <= 32), or -1.
 int il_log2(int nrp)
@@ -2411,7 +2527,7 @@ hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v
     if (n <= 0 || nr <= 0) return hipSuccess;
     const int tr = kTile >> lgp;
     hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v,
-                       (const double *)nullptr, x, ld, accum);
+                       (const double *)nullptr, x, ld, accum, 0);
     return hipGetLastError();
 }
 
@@ -2423,7 +2539,19 @@ hipError_t launch_deinterleave_sum(int n, int nr, int nrp, int flip, const doubl
     if (n <= 0 || nr <= 0) return hipSuccess;
     const int tr = kTile >> lgp;
     hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v, v2,
-                       x, ld, 0);
+                       x, ld, 0, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave_pairs(int n, int nr, int nrp, int flip, const double *v, double *x, int ld,
+                                     hipStream_t s)
+{
+    const int lgp = il_log2(nrp);
+    if (lgp < 0 || nr > nrp) return hipErrorInvalidValue;
+    if (n <= 0 || nr <= 0) return hipSuccess;
+    const int tr = kTile >> lgp;
+    hipLaunchKernelGGL(k_deinterleave, dim3((unsigned)((n + tr - 1) / tr)), dim3(256), 0, s, n, nr, lgp, flip, v,
+                       (const double *)nullptr, x, ld, 0, 1);
     return hipGetLastError();
 }
 

@@ -216,14 +216,16 @@ hipError_t launch_residual(int P, const DevPart *parts, const int *ia, const int
 // Level-scheduled substitution over the lower-form CSR (diagonal last).
 struct LevelArgs {
     // per level-ordered position q (levels are contiguous ranges of q): int4
-    // {row, first entry, end entry, 0} into the strictly lower entries,
-    // stored in level order (columns lja, values lval / lval32)
+    // {row, first entry, end entry, CSR start} into the strictly lower
+    // entries, stored in level order (columns lja, values lval / lval32);
+    // CSR start is the row's first entry in the lower-form CSR (a64)
     const int *rec;
     const int *lja;
     const double *lval;     // FP64 values
     const double *invd;     // 1/diag per row
     const float *lval32;    // FP32 values / 1/diag (fp32)
     const float *invd32;
+    const double *a64;      // lower-form CSR values, diagonal last (fp32 factors)
     const double *b;
     double *x;
     int ldb, ldx, nr, nr_slots, n, flip, G, fp32;
@@ -250,6 +252,15 @@ hipError_t launch_level_il(int r0, int cnt, int nrp, const LevelArgs &a, const d
 // against 1.59), so the single-RHS path keeps it.
 hipError_t launch_sync_free(int npos, int n, int nrp, const LevelArgs &a, const double *bi, double *xi,
                             int *abort_word, hipStream_t s);
+// FP32 level solve with one FP64 refinement step of nrp interleaved
+// right-hand sides in one sync-free launch (after the not-ready fill of pub):
+// per row x0 = M^-1 b, r = b - A x0 and dx = M^-1 r, with the arithmetic of
+// launch_sync_free, launch_residual_il and launch_sync_free again, in one
+// pass over the levels.  pub (n * nrp words) receives each row's x0 and dx
+// as one packed 64-bit word; launch_deinterleave_pairs forms x0 + dx.
+// Needs a.fp32, a.lval32, a.invd32 and a.a64.
+hipError_t launch_sync_free_refined1(int npos, int n, int nrp, const LevelArgs &a, const double *bi, double *pub,
+                                     int *abort_word, hipStream_t s);
 // Column-major caller layout <-> interleaved lower-form layout.
 hipError_t launch_interleave(int n, int nr, int nrp, int flip, const double *src, int ld, double *dst, hipStream_t s);
 hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v, double *x, int ld, int accum,
@@ -258,6 +269,9 @@ hipError_t launch_deinterleave(int n, int nr, int nrp, int flip, const double *v
 // refinement correction), first nr columns.
 hipError_t launch_deinterleave_sum(int n, int nr, int nrp, int flip, const double *v, const double *v2, double *x,
                                    int ld, hipStream_t s);
+// x (caller layout) = x0 + dx from launch_sync_free_refined1's packed words.
+hipError_t launch_deinterleave_pairs(int n, int nr, int nrp, int flip, const double *v, double *x, int ld,
+                                     hipStream_t s);
 // r = b - A x in FP64 on interleaved vectors (nrp values per row, a power of
 // two <= 32) over the lower-form CSR (ia, ja, a; diagonal included).
 hipError_t launch_residual_il(int n, int nrp, const int *ia, const int *ja, const double *a, const double *bi,

@@ -187,6 +187,28 @@ def test_sync_free_matches_level_launches(native, monkeypatch, uplo, precision):
         assert relerr(X1[j], orc.trsv(uplo, m.a, m.ia, m.ja, B[j])) <= TOL
 
 
+def test_sync_free_refined_give_up_is_reported(native, monkeypatch):
+    """The FP32 level solve with one refinement step runs as one sync-free
+    launch (x0, the FP64 residual and the correction per row, published as
+    one packed word).  With a poll limit of 0 (forced from the host) its
+    give-up is reported as SPIKE_ERR_DEVICE, and every finite value is the
+    result of the full solve."""
+    m = gen.lap3d(32)
+    B = gen.rhs(m.n, 8, seed=gen.SEED_BASE + 182)
+    want = native.Factor("L", m.a, m.ia, m.ja, method="levels", precision="fp32_refined",
+                         refine_steps=1).solve(B, nrhs=8)
+    monkeypatch.setenv("SPIKE_PSTRSV_SPIN_LIMIT", "0")
+    f = native.Factor("L", m.a, m.ia, m.ja, method="levels", precision="fp32_refined", refine_steps=1)
+    X = np.zeros((8, m.n))
+    with pytest.raises(native.SpikeError) as ei:
+        f.solve(B, X, nrhs=8)
+    assert ei.value.code == 3, ei.value
+    fin = np.isfinite(X)
+    assert not fin.all(), "the give-up path did not run"
+    assert np.array_equal(X[fin], want[fin])
+    f.status()
+
+
 def test_sync_free_give_up_is_reported(native, monkeypatch):
     """With a poll limit of 0 (forced from the host) rows that find an
     operand unpublished give up at once.  The solve still ends, every finite
