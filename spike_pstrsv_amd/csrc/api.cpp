@@ -1857,9 +1857,33 @@ int solve_levels_refined1(spike_factor_s *f, const double *b, int ldb, double *x
     return SPIKE_OK;
 }
 
+// FP32 level solve of one right-hand side with one refinement step: one
+// launch per level computes x0 (into x), the FP64 residual and the
+// correction (into d_dlt) row by row, then x += d.
+int solve_levels_refined1_single(spike_factor_s *f, const double *b, double *x, hipStream_t s)
+{
+    spike::LevelArgs la = level_il_args(f);
+    la.b = b;
+    la.ldb = f->plan.n;
+    la.x = x;
+    la.ldx = f->plan.n;
+    la.nr = 1;
+    la.nr_slots = 1;
+    la.n = f->plan.n;
+    la.flip = f->plan.flip ? 1 : 0;
+    la.G = f->lev_G;
+    for (int l = 0; l + 1 < (int)f->lev_ptr.size(); l++)
+        HIP_TRY(spike::launch_level_ref1(f->lev_ptr[l], f->lev_ptr[l + 1] - f->lev_ptr[l], la, f->d_dlt, s));
+    HIP_TRY(spike::launch_axpy(f->plan.n, 1, f->d_dlt, f->plan.n, x, f->plan.n, s));
+    return SPIKE_OK;
+}
+
 int solve_chunk(spike_factor_s *f, const double *b, int ldb, double *x, int ldx, int nr, hipStream_t s,
                 hipEvent_t *ev, double *ph)
 {
+    if (f->method == SPIKE_METHOD_LEVELS && f->fp32 && f->refine_steps == 1 && nr == 1 && !ev && f->mats.empty() &&
+        f->d_dlt)
+        return solve_levels_refined1_single(f, b, x, s);
     // d_r holds n x 32 values on level factors with FP32 values
     if (f->method == SPIKE_METHOD_LEVELS && f->fp32 && f->refine_steps == 1 && nr >= 2 && nr <= 32 && !ev &&
         f->mats.empty() && f->d_r)

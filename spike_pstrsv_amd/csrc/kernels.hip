@@ -1442,6 +1442,79 @@ __global__ __launch_bounds__(256) void k_level(int r0, int cnt, const int4 *__re
     }
 }
 
+// Single-RHS level step of an FP32 level factor with one FP64 refinement
+// step, G lanes per row as in k_level<G, 1, float>: x0 = (b - L x0) / d
+// (k_level's operations, same bits), r = b - A x0 in FP64 (lane partial sums
+// over the row's entries, the group sum, then the diagonal term; rec.w is the
+// row's first entry in the lower-form CSR a64, diagonal last) and
+// dx = (r - L dx) / d.  x0 goes to x and dx to dl, both at the caller's
+// index; x += dl after the last level completes the step.  One launch per
+// level instead of two, plus no residual launch.
+template <int G>
+__global__ __launch_bounds__(256) void k_level_ref1(int r0, int cnt, const int4 *__restrict__ rec,
+                                                    const int *__restrict__ lja, const float *__restrict__ lval,
+                                                    const float *__restrict__ invd, const double *__restrict__ a64,
+                                                    const double *b, double *x, double *dl, int n, int flip)
+{
+    constexpr int U = 4;
+    const int gid = blockIdx.x * 256 + threadIdx.x;
+    const int q = gid / G, li = gid % G;
+    const int4 rc = q < cnt ? rec[r0 + q] : make_int4(-1, 0, 0, 0);
+    const bool ok = rc.x >= 0;   // padding records have row -1 and no entries
+    const int row = rc.x;
+    // 1/diag, b and the FP64 diagonal of the row (row 0 for padding, unused),
+    // loaded with the entries
+    const int rs = ok ? row : 0, gs = flip ? n - 1 - rs : rs;
+    const float d = invd[rs];
+    const double bv = b[gs];
+    const int abase = rc.w - rc.y;   // a64 index of level-ordered entry e: abase + e
+    const double ad = a64[ok ? abase + rc.z : 0];
+    float acc0 = 0.f, acc1 = 0.f;
+    double acc64 = 0.0;
+    for (int e = rc.y + li; e < rc.z; e += U * G) {
+        int c[U];
+        float w[U];
+        double w64[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // unconditional loads at a clamped index, masked entries then
+            // use the row itself (not used)
+            const bool in = e + u * G < rc.z;
+            const int ee = in ? e + u * G : rc.z - 1;
+            const int cl = lja[ee];
+            const float wl = lval[ee];
+            const double wl64 = a64[abase + ee];
+            const int cc = in ? cl : row;
+            c[u] = flip ? n - 1 - cc : cc;
+            w[u] = in ? wl : 0.f;
+            w64[u] = in ? wl64 : 0.0;
+        }
+        double xv[U], dv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            xv[u] = x[c[u]];
+            dv[u] = dl[c[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (e + u * G < rc.z) {
+                acc0 = fma(w[u], (float)xv[u], acc0);
+                acc64 = fma(w64[u], xv[u], acc64);
+                acc1 = fma(w[u], (float)dv[u], acc1);
+            }
+    }
+    acc0 = group_sum<G>(acc0);
+    acc64 = group_sum<G>(acc64);
+    acc1 = group_sum<G>(acc1);
+    const float x0 = ((float)bv - acc0) * d;
+    const double res = bv - fma(ad, (double)x0, acc64);
+    const float dx = ((float)res - acc1) * d;
+    if (ok && li == 0) {
+        x[gs] = (double)x0;
+        dl[gs] = (double)dx;
+    }
+}
+
 // Multi-RHS level step on interleaved vectors (v[i*NRP + r]): NRP lanes per
 // row, one per right-hand side, each summing the row's entries in order
 // (the entry's value and column are the same for all lanes of the row; the
@@ -2407,6 +2480,30 @@ hipError_t launch_level(int r0, int cnt, const LevelArgs &a, hipStream_t s)
     case 4: return launch_level_nr<4>(r0, cnt, a, s);
     case 8: return launch_level_nr<8>(r0, cnt, a, s);
     case 16: return launch_level_nr<16>(r0, cnt, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int G>
+static hipError_t level_ref1_g(int r0, int cnt, const LevelArgs &a, double *dl, hipStream_t s)
+{
+    const dim3 grid((unsigned)(((size_t)cnt * G + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_level_ref1<G>), grid, block, 0, s, r0, cnt, (const int4 *)a.rec, a.lja, a.lval32, a.invd32,
+                       a.a64, a.b, a.x, dl, a.n, a.flip);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_ref1(int r0, int cnt, const LevelArgs &a, double *dl, hipStream_t s)
+{
+    if (!a.fp32 || a.a64 == nullptr || a.lval32 == nullptr || a.invd32 == nullptr || dl == nullptr)
+        return hipErrorInvalidValue;
+    if (cnt <= 0) return hipSuccess;
+    switch (a.G) {
+    case 1: return level_ref1_g<1>(r0, cnt, a, dl, s);
+    case 2: return level_ref1_g<2>(r0, cnt, a, dl, s);
+    case 4: return level_ref1_g<4>(r0, cnt, a, dl, s);
+    case 8: return level_ref1_g<8>(r0, cnt, a, dl, s);
+    case 16: return level_ref1_g<16>(r0, cnt, a, dl, s);
     default: return hipErrorInvalidValue;
     }
 }

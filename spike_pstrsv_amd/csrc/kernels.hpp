@@ -237,6 +237,11 @@ struct LevelArgs {
 constexpr int kSpinLimit = 1 << 20;
 // Positions r0 .. r0+cnt (one level): x = (b - strict-lower part . x) / diag.
 hipError_t launch_level(int r0, int cnt, const LevelArgs &a, hipStream_t s);
+// Single-RHS level step of an FP32 level factor with one FP64 refinement
+// step (a.b, a.x of one column, a.fp32, a.lval32, a.invd32, a.a64): x0 into
+// a.x and the correction into dl (n values, caller-indexed); after the last
+// level x += dl finishes the solve.
+hipError_t launch_level_ref1(int r0, int cnt, const LevelArgs &a, double *dl, hipStream_t s);
 // Level step of nrp interleaved right-hand sides (v[i*nrp + r], nrp in
 // {2, 4, 8, 16, 32}): one lane per row and right-hand side.
 hipError_t launch_level_il(int r0, int cnt, int nrp, const LevelArgs &a, const double *bi, double *xi, hipStream_t s);

@@ -143,6 +143,31 @@ def test_levels_refined_interleaved(native, uplo, nrhs):
         assert relerr(X[j], orc.trsv(uplo, m.a, m.ia, m.ja, B.reshape(nrhs, m.n)[j])) <= TOL
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_levels_refined_single_rhs(native, uplo):
+    """One right-hand side of an FP32 level factor with one refinement step:
+    one launch per level does x0, the FP64 residual and the correction.  The
+    profiled solve runs the separate launches (x0 solve, residual launch,
+    correction solve); both meet the oracle bar and agree to rounding."""
+    import torch
+
+    m = gen.lap3d(48)
+    if uplo == "U":
+        m = gen.to_upper(m)
+    b = gen.rhs(m.n, 1, seed=gen.SEED_BASE + 185)
+    f = native.Factor(uplo, m.a, m.ia, m.ja, method="levels", precision="fp32_refined", refine_steps=1)
+    x = f.solve(b)
+    want = orc.trsv(uplo, m.a, m.ia, m.ja, b)
+    print("lap3d_48 levels fp32_refined 1 RHS", uplo, relerr(x, want))
+    assert relerr(x, want) <= TOL
+    db = torch.from_numpy(np.ascontiguousarray(b)).to("cuda:0")
+    dx = torch.zeros_like(db)
+    f.solve_profiled(db.data_ptr(), dx.data_ptr())
+    xp = dx.cpu().numpy()
+    assert relerr(xp, want) <= TOL
+    assert relerr(x, xp) <= 1e-13
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
 @pytest.mark.parametrize("nrhs", [1, 5, 11])
 def test_levels_forced_banded(native, precision, nrhs):
