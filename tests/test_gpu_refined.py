@@ -126,7 +126,7 @@ def test_levels_chosen_for_lap3d(native, uplo):
 
 
 @pytest.mark.parametrize("uplo", ["L", "U"])
-@pytest.mark.parametrize("nrhs", [3, 32])
+@pytest.mark.parametrize("nrhs", [2, 3, 17, 32])
 def test_levels_refined_interleaved(native, uplo, nrhs):
     """FP32 level solve with one refinement step of several right-hand sides:
     the residual and the correction stay interleaved between launches."""
