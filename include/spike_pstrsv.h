@@ -336,7 +336,7 @@ typedef struct {
     double window_us;         /* batching window after the oldest waiting      */
                               /* request; < 0 = 200                            */
     int max_batch_rhs;        /* right-hand sides merged into one solve, and   */
-                              /* the most one request may carry; 0 = 8         */
+                              /* the most one request may carry; 0 = 16        */
     double admit_timeout_ms;  /* spike_server_add of a matrix whose footprint  */
                               /* does not fit the remaining budget waits this  */
                               /* long for spike_server_remove to free memory;  */

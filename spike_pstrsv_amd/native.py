@@ -450,7 +450,7 @@ class Server:
     Requests carry device pointers (int addresses or tensors with data_ptr())."""
 
     def __init__(self, hbm_budget: int = 0, slots: int = 64, streams: int = 8, window_us: float = 200.0,
-                 max_batch_rhs: int = 8, admit_timeout_ms: float = -1.0):
+                 max_batch_rhs: int = 16, admit_timeout_ms: float = -1.0):
         o = ServerOptions()
         lib.spike_server_options_default(C.byref(o))
         o.hbm_budget = hbm_budget

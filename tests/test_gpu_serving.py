@@ -98,7 +98,7 @@ def test_serving_admission_by_footprint(native):
     srv.close()
 
 
-def _served_bytes(native, m, max_batch_rhs=8):
+def _served_bytes(native, m, max_batch_rhs=16):
     """Footprint the server admits for matrix m: the factor after its
     warm-up solves (every padded column count) plus the staging buffers."""
     import torch
